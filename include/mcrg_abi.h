@@ -1,0 +1,136 @@
+/*
+ * mcrg_abi.h -- C ABI of the MI355X MCRG hot path (libmcrg_amd.so).
+ *
+ * Plain C types only: opaque handles, pointers and sizes, int status codes.
+ * Nothing here throws; on failure a function returns a negative mcrg_status and
+ * mcrg_last_error() describes it (thread-local).
+ *
+ * The reference (kim-jane/MCRG) has no FFI: its boundary is three C++ classes.
+ * Each entry point below names the reference member it stands in for; the
+ * C++ classes that keep the reference's names sit on top of this ABI
+ * (mcrg_amd/csrc/host/).
+ *
+ * Model and stream conventions: coupling J = |K| > 0 is ferromagnetic (the
+ * reference writes K = -J, src/ising.cpp:9); spins are +1/-1 int8 in row-major
+ * order, site (x, y) at y*L + x (the reference's column-major (i, j) at j*N + i);
+ * one "sweep" is a red/black checkerboard Metropolis pass driven by
+ * Philox4x32-10 keyed by the seed.  Operator sums are single-counted (the
+ * reference's calc_interactions returns twice NN and NNN).
+ */
+#ifndef MCRG_ABI_H
+#define MCRG_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCRG_ABI_VERSION 1
+
+typedef int32_t mcrg_status;
+enum {
+    MCRG_OK = 0,
+    MCRG_ERR_INVALID_ARG = -1,   /* bad size, coupling, level count, handle ... */
+    MCRG_ERR_OUT_OF_BUDGET = -2, /* device memory or admission budget exhausted */
+    MCRG_ERR_DEVICE = -3,        /* a HIP call failed */
+    MCRG_ERR_NOT_READY = -4,     /* result not available yet */
+    MCRG_ERR_NOT_FOUND = -5,     /* unknown job id */
+    MCRG_ERR_CAPACITY = -6       /* record bins exhausted */
+};
+
+enum { MCRG_LAYOUT_INT8 = 0, MCRG_LAYOUT_PACKED = 1 };
+
+typedef struct mcrg_ctx_s *mcrg_ctx;
+typedef struct mcrg_chain_s *mcrg_chain;
+
+/* Library version and last error of the calling thread. */
+int32_t mcrg_abi_version(void);
+const char *mcrg_last_error(void);
+
+/* Metropolis thresholds floor(exp(-4J q) 2^32), q = 1, 2 (a value of 2^32 means
+ * "always flip").  Replaces P_ = 1 - exp(2K) of IsingModel(double K),
+ * src/ising.cpp:9. */
+mcrg_status mcrg_thresholds(double coupling, uint64_t thr[2]);
+
+/* int64 entries in one per-bin record for n_levels blockings:
+ * 21 n_levels + 16.  Layout (l = 0..n_levels, a, b in {NN, NNN, P}):
+ *   [l*3 + a]                                 sum S_l[a]
+ *   [3(nl+1) + l*9 + b*3 + a]                 sum S_l[a] S_l[b]
+ *   [12(nl+1) + (l-1)*9 + b*3 + a], l >= 1    sum S_l[a] S_{l-1}[b]
+ *   [12(nl+1) + 9nl + 0..3]                   sum M, sum |M|, sum M^2, samples
+ * These are the accumulators S_avg_loc, Sb_Sb_avg_loc, Sb_S_avg_loc of
+ * calc_critical_exponent (src/mcrg.cpp:59-66, flattened as
+ * src/definitions.cpp:9-19), kept exactly in int64 per bin. */
+int32_t mcrg_record_size(int32_t n_levels);
+
+/* A context owns one device.  Replaces the MPI_Init / MPI_COMM_WORLD setup of
+ * src/main.cpp:6 (one process per device; no MPI). */
+mcrg_status mcrg_ctx_create(int32_t device, mcrg_ctx *out);
+mcrg_status mcrg_ctx_destroy(mcrg_ctx ctx);
+
+/* ------------------------------------------------------------------ chains */
+typedef struct {
+    int32_t L;              /* side, power of two, 4 <= L <= 16384          */
+    double coupling;        /* J = |K| >= 0                                  */
+    uint64_t seed;          /* Philox key of this chain                      */
+    int32_t n_levels;       /* blockings per sample, 0 <= n <= log2(L) - 1   */
+    int32_t layout;         /* MCRG_LAYOUT_INT8 or MCRG_LAYOUT_PACKED        */
+    int32_t sweeps_per_bin; /* measured sweeps per record bin (>= 1)         */
+    int32_t max_bins;       /* record bins allocated (>= 1)                  */
+} mcrg_chain_desc;
+
+/* Allocates the chain on the device and draws the Philox +-1 start.
+ * Replaces Lattice(int N) + initialize_random_spins (src/lattice.cpp:3-15,33-41)
+ * together with IsingModel(double K) (src/ising.cpp:3-11). */
+mcrg_status mcrg_chain_create(mcrg_ctx ctx, const mcrg_chain_desc *desc, mcrg_chain *out);
+mcrg_status mcrg_chain_destroy(mcrg_chain ch);
+
+/* Changes the coupling for subsequent sweeps (IsingModel(double K) applied to an
+ * existing Lattice, src/mcrg.cpp:46-50). */
+mcrg_status mcrg_chain_set_coupling(mcrg_chain ch, double coupling);
+
+/* Enqueues n_sweeps sweeps (asynchronous).  With measure != 0 every sweep is
+ * followed by the level 0..n_levels measurement, accumulated into the record
+ * of bin (sample index / sweeps_per_bin).  One sweep stands in for one
+ * IsingModel::sample_new_configuration (src/ising.cpp:87-93); a run of
+ * unmeasured sweeps is IsingModel::equilibrate(.., write = false)
+ * (src/ising.cpp:14-84); measured sweeps are the sampling loop of
+ * calc_critical_exponent (src/mcrg.cpp:72-98). */
+mcrg_status mcrg_chain_sweep(mcrg_chain ch, int32_t n_sweeps, int32_t measure);
+
+/* Waits for the chain's queued work. */
+mcrg_status mcrg_chain_sync(mcrg_chain ch);
+
+/* Spins as +-1 int8, row-major, n = L*L (synchronous).  Backs the public
+ * Lattice::spins_ (src/lattice.hpp:16) and checkpoint/restore. */
+mcrg_status mcrg_chain_get_spins(mcrg_chain ch, int8_t *host, int64_t n);
+mcrg_status mcrg_chain_set_spins(mcrg_chain ch, const int8_t *host, int64_t n);
+
+/* All max_bins records, n = max_bins * mcrg_record_size(n_levels) (synchronous). */
+mcrg_status mcrg_chain_get_records(mcrg_chain ch, int64_t *host, int64_t n);
+
+/* Sweep counter t (next sweep index) and number of measured samples. */
+mcrg_status mcrg_chain_get_state(mcrg_chain ch, uint32_t *sweep, uint32_t *n_measured);
+/* Restores counters and records (resume from a checkpoint); n as get_records. */
+mcrg_status mcrg_chain_set_state(mcrg_chain ch, uint32_t sweep, uint32_t n_measured,
+                                 const int64_t *records, int64_t n);
+
+/* Single-count NN, NNN, plaquette and magnetisation of the current spins
+ * (synchronous).  Lattice::calc_interactions (src/lattice.cpp:102-120) returns
+ * 2*out[0], 2*out[1]; IsingModel::calc_magnetization (src/ising.cpp:176-179)
+ * is out[3] / L^2. */
+mcrg_status mcrg_chain_operators(mcrg_chain ch, int64_t out[4]);
+
+/* ------------------------------------------------------------------ timing */
+/* Kernel-duration statistics measured with HIP events on the stream that
+ * launched the kernel, enabled per context.  kind: 0 = half-sweep kernel,
+ * 1 = level-0 measurement kernel, 2 = levels/accumulate kernel,
+ * 3 = batched resident kernel. */
+mcrg_status mcrg_ctx_set_timing(mcrg_ctx ctx, int32_t enable);
+mcrg_status mcrg_ctx_kernel_time(mcrg_ctx ctx, int32_t kind, int64_t *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

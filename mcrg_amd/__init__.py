@@ -1,0 +1,195 @@
+"""mcrg_amd -- MI355X-native MCRG hot path (checkerboard Metropolis, b=2 majority
+blocking, exact int64 even-operator correlations) behind a C ABI.
+
+Python here is a thin ctypes view of libmcrg_amd.so for tests and benchmarks;
+the hot path is the HIP code in csrc/.  Importing this package raises if the
+library has not been built (``make -C mcrg_amd``): there is no fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmcrg_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it with `make -C {_HERE}` (hipcc, gfx950)")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+OK = 0
+ERR_INVALID_ARG = -1
+ERR_OUT_OF_BUDGET = -2
+ERR_DEVICE = -3
+ERR_NOT_READY = -4
+ERR_NOT_FOUND = -5
+ERR_CAPACITY = -6
+LAYOUT_INT8 = 0
+LAYOUT_PACKED = 1
+
+
+class MCRGError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"mcrg status {status}: {msg}")
+        self.status = status
+
+
+class ChainDesc(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_int32), ("coupling", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("n_levels", ctypes.c_int32), ("layout", ctypes.c_int32),
+                ("sweeps_per_bin", ctypes.c_int32), ("max_bins", ctypes.c_int32)]
+
+
+_c = ctypes
+_p = ctypes.c_void_p
+_SIGS = {
+    "mcrg_abi_version": (_c.c_int32, []),
+    "mcrg_last_error": (_c.c_char_p, []),
+    "mcrg_thresholds": (_c.c_int32, [_c.c_double, _c.POINTER(_c.c_uint64)]),
+    "mcrg_record_size": (_c.c_int32, [_c.c_int32]),
+    "mcrg_ctx_create": (_c.c_int32, [_c.c_int32, _c.POINTER(_p)]),
+    "mcrg_ctx_destroy": (_c.c_int32, [_p]),
+    "mcrg_ctx_set_timing": (_c.c_int32, [_p, _c.c_int32]),
+    "mcrg_ctx_kernel_time": (_c.c_int32, [_p, _c.c_int32, _c.POINTER(_c.c_int64), _c.POINTER(_c.c_double)]),
+    "mcrg_chain_create": (_c.c_int32, [_p, _c.POINTER(ChainDesc), _c.POINTER(_p)]),
+    "mcrg_chain_destroy": (_c.c_int32, [_p]),
+    "mcrg_chain_set_coupling": (_c.c_int32, [_p, _c.c_double]),
+    "mcrg_chain_sweep": (_c.c_int32, [_p, _c.c_int32, _c.c_int32]),
+    "mcrg_chain_sync": (_c.c_int32, [_p]),
+    "mcrg_chain_get_spins": (_c.c_int32, [_p, _p, _c.c_int64]),
+    "mcrg_chain_set_spins": (_c.c_int32, [_p, _p, _c.c_int64]),
+    "mcrg_chain_get_records": (_c.c_int32, [_p, _p, _c.c_int64]),
+    "mcrg_chain_get_state": (_c.c_int32, [_p, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint32)]),
+    "mcrg_chain_set_state": (_c.c_int32, [_p, _c.c_uint32, _c.c_uint32, _p, _c.c_int64]),
+    "mcrg_chain_operators": (_c.c_int32, [_p, _c.POINTER(_c.c_int64)]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def lib():
+    return _lib
+
+
+def last_error():
+    return _lib.mcrg_last_error().decode()
+
+
+def check(status):
+    if status != OK:
+        raise MCRGError(status, last_error())
+    return status
+
+
+def abi_version():
+    return _lib.mcrg_abi_version()
+
+
+def thresholds(coupling):
+    thr = (ctypes.c_uint64 * 2)()
+    check(_lib.mcrg_thresholds(float(coupling), thr))
+    return np.array(list(thr), dtype=np.uint64)
+
+
+def record_size(n_levels):
+    return _lib.mcrg_record_size(n_levels)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One device (mcrg_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(_lib.mcrg_ctx_create(device, ctypes.byref(h)))
+        self.handle = h
+
+    def set_timing(self, enable=True):
+        check(_lib.mcrg_ctx_set_timing(self.handle, 1 if enable else 0))
+
+    def kernel_time(self, kind):
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        check(_lib.mcrg_ctx_kernel_time(self.handle, kind, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def close(self):
+        if self.handle:
+            _lib.mcrg_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Chain:
+    """One Markov chain on the device (mcrg_chain)."""
+
+    def __init__(self, ctx, L, coupling, seed, n_levels=0, layout=LAYOUT_INT8,
+                 sweeps_per_bin=1000, max_bins=1):
+        d = ChainDesc(L, float(coupling), seed, n_levels, layout, sweeps_per_bin, max_bins)
+        h = ctypes.c_void_p()
+        check(_lib.mcrg_chain_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)))
+        self.handle = h
+        self.L, self.n_levels, self.max_bins = L, n_levels, max_bins
+
+    def sweep(self, n, measure=False):
+        check(_lib.mcrg_chain_sweep(self.handle, n, 1 if measure else 0))
+
+    def sync(self):
+        check(_lib.mcrg_chain_sync(self.handle))
+
+    def set_coupling(self, coupling):
+        check(_lib.mcrg_chain_set_coupling(self.handle, float(coupling)))
+
+    def spins(self):
+        a = np.zeros(self.L * self.L, dtype=np.int8)
+        check(_lib.mcrg_chain_get_spins(self.handle, _ptr(a), a.size))
+        return a
+
+    def set_spins(self, s):
+        a = np.ascontiguousarray(s, dtype=np.int8).reshape(-1)
+        check(_lib.mcrg_chain_set_spins(self.handle, _ptr(a), a.size))
+
+    def records(self):
+        rs = record_size(self.n_levels)
+        a = np.zeros(self.max_bins * rs, dtype=np.int64)
+        check(_lib.mcrg_chain_get_records(self.handle, _ptr(a), a.size))
+        return a.reshape(self.max_bins, rs)
+
+    def state(self):
+        t, q = ctypes.c_uint32(), ctypes.c_uint32()
+        check(_lib.mcrg_chain_get_state(self.handle, ctypes.byref(t), ctypes.byref(q)))
+        return t.value, q.value
+
+    def set_state(self, sweep, n_measured, records=None):
+        if records is None:
+            check(_lib.mcrg_chain_set_state(self.handle, sweep, n_measured, None, 0))
+        else:
+            r = np.ascontiguousarray(records, dtype=np.int64).reshape(-1)
+            check(_lib.mcrg_chain_set_state(self.handle, sweep, n_measured, _ptr(r), r.size))
+
+    def operators(self):
+        out = (ctypes.c_int64 * 4)()
+        check(_lib.mcrg_chain_operators(self.handle, out))
+        return np.array(list(out), dtype=np.int64)
+
+    def close(self):
+        if self.handle:
+            _lib.mcrg_chain_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

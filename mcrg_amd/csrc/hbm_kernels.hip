@@ -1,0 +1,459 @@
+// HBM-resident schedule: int8 lattices in global memory, one launch per
+// half-sweep, then two launches per measurement.  Used for the plain
+// single-lattice path and for batches of large lattices (one grid row per
+// lattice, blockIdx.y).
+//
+//   k_int8_init           Philox +-1 start        (Lattice::initialize_random_spins, src/lattice.cpp:33-41)
+//   k_int8_half_sweep     checkerboard Metropolis (replaces Wolff, src/ising.cpp:87-149)
+//   k_int8_measure0       level-0 NN/NNN/P + M and b=2 majority blocking into a
+//                         bit-packed level-1 lattice (src/lattice.cpp:102-120, src/mcrg.cpp:314-348)
+//   k_levels_accumulate   levels 1..nl in one workgroup per lattice, then the
+//                         per-sample sums and products into the bin record
+//                         (src/mcrg.cpp:72-98)
+#include "device_common.hpp"
+#include "hbm_kernels.hpp"
+
+namespace mcrg {
+
+// ---------------------------------------------------------------- init
+__global__ __launch_bounds__(256) void k_int8_init(const HbmLattice *__restrict__ lats)
+{
+    const HbmLattice &lat = lats[blockIdx.y];
+    const uint32_t L = (uint32_t)lat.L;
+    const uint32_t n = L * L;
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const u32x4 r = philox(m >> 7, 0u, 0u, kTagInit << 28, lat.k0, lat.k1);
+    const uint32_t bit = (pick4(r, (m >> 5) & 3u) >> (m & 31u)) & 1u;
+    lat.spins[m] = (int8_t)(1 - 2 * (int)bit);
+}
+
+// ---------------------------------------------------------------- sweep
+// One thread per word of 32 same-colour sites.  For L >= 64 a word is the
+// sites of one colour in a 64-byte run of one row.
+__device__ __forceinline__ void load64(const int8_t *p, uint32_t d[16])
+{
+    const uint4 *v = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint4 x = v[i];
+        d[4 * i + 0] = x.x;
+        d[4 * i + 1] = x.y;
+        d[4 * i + 2] = x.z;
+        d[4 * i + 3] = x.w;
+    }
+}
+
+__device__ __forceinline__ void store64(int8_t *p, const uint32_t d[16])
+{
+    uint4 *v = reinterpret_cast<uint4 *>(p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = make_uint4(d[4 * i], d[4 * i + 1], d[4 * i + 2], d[4 * i + 3]);
+}
+
+__device__ __forceinline__ uint32_t sbit(int8_t v) { return ((uint32_t)(uint8_t)v) >> 7; }
+
+__global__ __launch_bounds__(256) void k_int8_half_sweep(const HbmLattice *__restrict__ lats,
+                                                         uint32_t t_off, uint32_t colour)
+{
+    const HbmLattice &lat = lats[blockIdx.y];
+    const uint32_t L = (uint32_t)lat.L;
+    const uint32_t t = lat.state[0] + t_off;
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    const Accept acc = lat.acc;
+    int8_t *spins = lat.spins;
+    if (L >= 64) {
+        const uint32_t wpr = L >> 6;
+        if (w >= L * wpr) return;
+        const uint32_t y = w / wpr, q = w % wpr;
+        const uint32_t p = (y + colour) & 1u;
+        int8_t *row = spins + (size_t)y * L + 64u * q;
+        const int8_t *up = spins + (size_t)((y - 1u) & (L - 1u)) * L + 64u * q;
+        const int8_t *dn = spins + (size_t)((y + 1u) & (L - 1u)) * L + 64u * q;
+        uint32_t R[16], U[16], D[16];
+        load64(row, R);
+        load64(up, U);
+        load64(dn, D);
+        const int8_t *rowbase = spins + (size_t)y * L;
+        const uint32_t s = gather_parity(R, p);
+        const uint32_t o = gather_parity(R, p ^ 1u);
+        const uint32_t nu = gather_parity(U, p);
+        const uint32_t nd = gather_parity(D, p);
+        uint32_t nl, nr;
+        if (p == 0u) {
+            const uint32_t e = sbit(rowbase[(64u * q - 1u) & (L - 1u)]);
+            nr = o;
+            nl = (o << 1) | e;
+        } else {
+            const uint32_t e = sbit(rowbase[(64u * q + 64u) & (L - 1u)]);
+            nl = o;
+            nr = (o >> 1) | (e << 31);
+        }
+        const uint32_t flip = metropolis_word(s, nu, nd, nl, nr, 0xFFFFFFFFu, acc, lat.k0, lat.k1,
+                                              w, t, colour);
+        if (flip) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const uint32_t f0 = (flip >> (2 * i)) & 1u, f1 = (flip >> (2 * i + 1)) & 1u;
+                R[i] ^= ((f0 * 0xFEu) << (8u * p)) | ((f1 * 0xFEu) << (8u * p + 16u));
+            }
+            store64(row, R);
+        }
+        return;
+    }
+    // L < 64: a word spans several rows; site-by-site.
+    const uint32_t half = L >> 1;
+    const uint32_t n_sites = L * half;
+    const uint32_t n_words = (n_sites + 31u) >> 5;
+    if (w >= n_words) return;
+    uint32_t s = 0, nu = 0, nd = 0, nl = 0, nr = 0, valid = 0;
+    for (uint32_t j = 0; j < 32u; ++j) {
+        const uint32_t m = 32u * w + j;
+        if (m >= n_sites) break;
+        const uint32_t y = m / half, k = m % half;
+        const uint32_t x = 2u * k + ((y + colour) & 1u);
+        const uint32_t ym = (y - 1u) & (L - 1u), yp = (y + 1u) & (L - 1u);
+        const uint32_t xm = (x - 1u) & (L - 1u), xp = (x + 1u) & (L - 1u);
+        valid |= 1u << j;
+        s |= sbit(spins[y * L + x]) << j;
+        nu |= sbit(spins[ym * L + x]) << j;
+        nd |= sbit(spins[yp * L + x]) << j;
+        nl |= sbit(spins[y * L + xm]) << j;
+        nr |= sbit(spins[y * L + xp]) << j;
+    }
+    const uint32_t flip = metropolis_word(s, nu, nd, nl, nr, valid, acc, lat.k0, lat.k1, w, t, colour);
+    for (uint32_t j = 0; j < 32u; ++j) {
+        if (!((flip >> j) & 1u)) continue;
+        const uint32_t m = 32u * w + j;
+        const uint32_t y = m / half, k = m % half;
+        const uint32_t x = 2u * k + ((y + colour) & 1u);
+        spins[y * L + x] = (int8_t)(-spins[y * L + x]);
+    }
+}
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Adds four per-thread counts into sums[0..3] with one atomic per wave.
+__device__ __forceinline__ void add4(unsigned long long *sums, uint32_t a, uint32_t b, uint32_t c,
+                                     uint32_t d)
+{
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    d = wave_sum(d);
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicAdd(sums + 0, (unsigned long long)a);
+        atomicAdd(sums + 1, (unsigned long long)b);
+        atomicAdd(sums + 2, (unsigned long long)c);
+        atomicAdd(sums + 3, (unsigned long long)d);
+    }
+}
+
+// ---------------------------------------------------------------- level 0
+// Disagreement counts: dNN = #{bonds with unequal spins} over the 2L^2 nearest
+// bonds, dNNN likewise over the 2L^2 diagonal bonds, dP = #{plaquettes with an
+// odd number of -1 spins}, ndown = #{-1 spins}.  Then NN = 2L^2 - 2 dNN,
+// NNN = 2L^2 - 2 dNNN, P = L^2 - 2 dP, M = L^2 - 2 ndown.
+__device__ __forceinline__ void row_bits64(const int8_t *p, uint32_t &lo, uint32_t &hi)
+{
+    uint32_t d[16];
+    load64(p, d);
+    lo = 0;
+    hi = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) lo |= sign4(d[i]) << (4 * i);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) hi |= sign4(d[8 + i]) << (4 * i);
+}
+
+__global__ __launch_bounds__(256) void k_int8_measure0(const HbmLattice *__restrict__ lats,
+                                                       uint32_t t_off)
+{
+    const HbmLattice &lat = lats[blockIdx.y];
+    const uint32_t L = (uint32_t)lat.L;
+    const uint32_t t = lat.state[0] + t_off;
+    const int8_t *spins = lat.spins;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t dnn = 0, dnnn = 0, dp = 0, ndown = 0;
+    if (L >= 64) {
+        const uint32_t wpr = L >> 6;  // 64-site chunks per row == level-1 words per row
+        const uint32_t n_items = (L >> 1) * wpr;
+        if (i < n_items) {
+            const uint32_t by = i / wpr, q = i % wpr;
+            uint32_t lo[3], hi[3], nx[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const uint32_t y = (2u * by + (uint32_t)r) & (L - 1u);
+                const int8_t *rowp = spins + (size_t)y * L;
+                row_bits64(rowp + 64u * q, lo[r], hi[r]);
+                nx[r] = sbit(rowp[(64u * q + 64u) & (L - 1u)]);
+            }
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                // right = s(x+1): shift the 64-bit row right by one, bring in the next chunk's bit.
+                const uint32_t alo = lo[r], ahi = hi[r], dlo = lo[r + 1], dhi = hi[r + 1];
+                const uint32_t rlo = (alo >> 1) | (ahi << 31), rhi = (ahi >> 1) | (nx[r] << 31);
+                const uint32_t glo = (dlo >> 1) | (dhi << 31), ghi = (dhi >> 1) | (nx[r + 1] << 31);
+                dnn += popc(alo ^ rlo) + popc(ahi ^ rhi) + popc(alo ^ dlo) + popc(ahi ^ dhi);
+                dnnn += popc(alo ^ glo) + popc(ahi ^ ghi) + popc(rlo ^ dlo) + popc(rhi ^ dhi);
+                dp += popc(alo ^ rlo ^ dlo ^ glo) + popc(ahi ^ rhi ^ dhi ^ ghi);
+                ndown += popc(alo) + popc(ahi);
+            }
+            // b = 2 blocks: even/odd columns of rows 2by and 2by+1.
+            const uint32_t coin = tie_coin_word(i, t, 0u, lat.k0, lat.k1);
+            uint32_t out = 0;
+            out = block16(lo[0], lo[1], coin & 0xFFFFu) | (block16(hi[0], hi[1], coin >> 16) << 16);
+            lat.lvl1[i] = out;
+        }
+    } else {
+        // L < 64: one thread per level-1 word, site by site.
+        const uint32_t Lb = L >> 1;
+        const uint32_t nb = Lb * Lb;
+        const uint32_t n_words = (nb + 31u) >> 5;
+        if (i < n_words) {
+            uint32_t out = 0;
+            const uint32_t coin = tie_coin_word(i, t, 0u, lat.k0, lat.k1);
+            for (uint32_t j = 0; j < 32u; ++j) {
+                const uint32_t bm = 32u * i + j;
+                if (bm >= nb) break;
+                const uint32_t by = bm / Lb, bx = bm % Lb;
+                uint32_t nd = 0;
+                for (uint32_t v = 0; v < 2u; ++v) {
+                    for (uint32_t u = 0; u < 2u; ++u) {
+                        const uint32_t x = 2u * bx + u, y = 2u * by + v;
+                        const uint32_t xp = (x + 1u) & (L - 1u), yp = (y + 1u) & (L - 1u);
+                        const uint32_t a = sbit(spins[y * L + x]), b = sbit(spins[y * L + xp]);
+                        const uint32_t c = sbit(spins[yp * L + x]), d = sbit(spins[yp * L + xp]);
+                        dnn += (a ^ b) + (a ^ c);
+                        dnnn += (a ^ d) + (b ^ c);
+                        dp += a ^ b ^ c ^ d;
+                        ndown += a;
+                        nd += a;
+                    }
+                }
+                const uint32_t bit = nd >= 3u ? 1u : nd <= 1u ? 0u : ((coin >> j) & 1u);
+                out |= bit << j;
+            }
+            lat.lvl1[i] = out;
+        }
+    }
+    add4(lat.sums0, dnn, dnnn, dp, ndown);
+}
+
+// ---------------------------------------------------------------- levels >= 1
+// Bit-packed row-major lattice ("RM"): site (x, y) of an Ll x Ll level at bit
+// m = y*Ll + x, word m >> 5.  Rows shorter than 32 bits share words.
+
+// Row y of a level with Ll <= 32 in the low Ll bits.
+__device__ __forceinline__ uint32_t rm_row_small(const uint32_t *buf, uint32_t Ll, uint32_t y)
+{
+    const uint32_t m = y * Ll;
+    const uint32_t mask = Ll >= 32u ? 0xFFFFFFFFu : ((1u << Ll) - 1u);
+    return (buf[m >> 5] >> (m & 31u)) & mask;
+}
+
+__device__ __forceinline__ uint32_t rot_right_small(uint32_t v, uint32_t Ll)
+{
+    // bit x of the result = bit x+1 (mod Ll) of v: the "right neighbour" row.
+    return ((v >> 1) | ((v & 1u) << (Ll - 1u))) & ((1u << Ll) - 1u);
+}
+
+// Disagreement counts of one level, summed over the workgroup into cnt[0..2].
+__device__ void rm_ops(const uint32_t *buf, uint32_t Ll, uint32_t *cnt)
+{
+    uint32_t dnn = 0, dnnn = 0, dp = 0;
+    if (Ll >= 32u) {
+        const uint32_t wpr = Ll >> 5;
+        const uint32_t n = Ll * wpr;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint32_t y = i / wpr, q = i % wpr;
+            const uint32_t yn = (y + 1u) & (Ll - 1u), qn = (q + 1u) % wpr;
+            const uint32_t a = buf[y * wpr + q], an = buf[y * wpr + qn];
+            const uint32_t d = buf[yn * wpr + q], dn = buf[yn * wpr + qn];
+            const uint32_t r = (a >> 1) | (an << 31), g = (d >> 1) | (dn << 31);
+            dnn += popc(a ^ r) + popc(a ^ d);
+            dnnn += popc(a ^ g) + popc(r ^ d);
+            dp += popc(a ^ r ^ d ^ g);
+        }
+    } else {
+        for (uint32_t y = threadIdx.x; y < Ll; y += blockDim.x) {
+            const uint32_t a = rm_row_small(buf, Ll, y);
+            const uint32_t d = rm_row_small(buf, Ll, (y + 1u) & (Ll - 1u));
+            const uint32_t r = rot_right_small(a, Ll), g = rot_right_small(d, Ll);
+            dnn += popc(a ^ r) + popc(a ^ d);
+            dnnn += popc(a ^ g) + popc(r ^ d);
+            dp += popc(a ^ r ^ d ^ g);
+        }
+    }
+    atomicAdd(cnt + 0, dnn);
+    atomicAdd(cnt + 1, dnnn);
+    atomicAdd(cnt + 2, dp);
+}
+
+// b = 2 majority blocking of an RM level (size Ll) into dst (size Ll/2).
+__device__ void rm_block(const uint32_t *src, uint32_t Ll, uint32_t *dst, uint32_t t,
+                         uint32_t level, uint32_t k0, uint32_t k1)
+{
+    const uint32_t Lb = Ll >> 1;
+    const uint32_t nb = Lb * Lb;
+    const uint32_t n_out = (nb + 31u) >> 5;
+    for (uint32_t i = threadIdx.x; i < n_out; i += blockDim.x) {
+        const uint32_t coin = tie_coin_word(i, t, level, k0, k1);
+        uint32_t out = 0;
+        if (Lb >= 32u) {
+            const uint32_t wpo = Lb >> 5, wpi = Ll >> 5;
+            const uint32_t by = i / wpo, q = i % wpo;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t w0 = src[(2u * by) * wpi + 2u * q + h];
+                const uint32_t w1 = src[(2u * by + 1u) * wpi + 2u * q + h];
+                out |= block16(w0, w1, (coin >> (16 * h)) & 0xFFFFu) << (16 * h);
+            }
+        } else {
+            // rows of Lb < 32 bits: this word holds blocked rows by0 .. by0 + 32/Lb - 1
+            const uint32_t rows = (32u / Lb) < Lb ? (32u / Lb) : Lb;
+            const uint32_t by0 = (32u * i) / Lb;
+            for (uint32_t rr = 0; rr < rows; ++rr) {
+                const uint32_t by = by0 + rr;
+                if (by >= Lb) break;
+                const uint32_t w0 = rm_row_small(src, Ll, 2u * by), w1 = rm_row_small(src, Ll, 2u * by + 1u);
+                const uint32_t off = (by * Lb) & 31u;
+                const uint32_t mask = (1u << Lb) - 1u;
+                out |= (block16(w0, w1, coin >> off) & mask) << off;
+            }
+        }
+        dst[i] = out;
+    }
+}
+
+// One workgroup per lattice.  Levels 1..nl: level 1 is read from global memory
+// (written by k_int8_measure0), deeper levels live in LDS.  Thread 0 then turns
+// the counts into operator values and adds the sample to its bin record.
+__global__ __launch_bounds__(1024) void k_levels_accumulate(const HbmLattice *__restrict__ lats,
+                                                            uint32_t t_off, uint32_t q_off)
+{
+    const HbmLattice &lat = lats[blockIdx.x];
+    const uint32_t L = (uint32_t)lat.L;
+    const uint32_t nl = (uint32_t)lat.n_levels;
+    const uint32_t t = lat.state[0] + t_off;
+    __shared__ uint32_t cnt[kMaxLevels + 1][3];
+    __shared__ uint32_t bufA[kLdsLevelWords];
+    __shared__ uint32_t bufB[kLdsLevelWords / 4];
+    for (uint32_t i = threadIdx.x; i < (kMaxLevels + 1) * 3; i += blockDim.x) (&cnt[0][0])[i] = 0u;
+    __syncthreads();
+    const uint32_t *cur = lat.lvl1;
+    uint32_t Ll = L >> 1;
+    for (uint32_t l = 1; l <= nl; ++l) {
+        rm_ops(cur, Ll, cnt[l]);
+        if (l < nl) {
+            uint32_t *dst = (l & 1u) ? bufA : bufB;
+            __syncthreads();
+            rm_block(cur, Ll, dst, t, l, lat.k0, lat.k1);
+            __syncthreads();
+            cur = dst;
+            Ll >>= 1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t S[kMaxLevels + 1][3];
+        unsigned long long *s0 = lat.sums0;
+        const int64_t n0 = (int64_t)L * L;
+        S[0][0] = 2 * n0 - 2 * (int64_t)s0[0];
+        S[0][1] = 2 * n0 - 2 * (int64_t)s0[1];
+        S[0][2] = n0 - 2 * (int64_t)s0[2];
+        const int64_t M = n0 - 2 * (int64_t)s0[3];
+        s0[0] = 0ull;
+        s0[1] = 0ull;
+        s0[2] = 0ull;
+        s0[3] = 0ull;
+        for (uint32_t l = 1; l <= nl; ++l) {
+            const int64_t nlv = (int64_t)(L >> l) * (L >> l);
+            S[l][0] = 2 * nlv - 2 * (int64_t)cnt[l][0];
+            S[l][1] = 2 * nlv - 2 * (int64_t)cnt[l][1];
+            S[l][2] = nlv - 2 * (int64_t)cnt[l][2];
+        }
+        const uint32_t q = lat.state[1] + q_off;
+        const uint32_t bin = q / (uint32_t)lat.sweeps_per_bin;
+        const uint32_t rs = record_size(nl);
+        int64_t *rec = lat.records + (size_t)bin * rs;
+        for (uint32_t l = 0; l <= nl; ++l) {
+            for (uint32_t a = 0; a < 3u; ++a) {
+                rec[off_S(l, a)] += S[l][a];
+                for (uint32_t b = 0; b < 3u; ++b) {
+                    rec[off_self(nl, l, a, b)] += S[l][a] * S[l][b];
+                    if (l >= 1u) rec[off_cross(nl, l, a, b)] += S[l][a] * S[l - 1][b];
+                }
+            }
+        }
+        const uint32_t om = off_M(nl);
+        rec[om + 0] += M;
+        rec[om + 1] += M < 0 ? -M : M;
+        rec[om + 2] += M * M;
+        rec[om + 3] += 1;
+    }
+}
+
+// Advances the chain's sweep counter and sample counter after a chunk.
+__global__ void k_advance(const HbmLattice *__restrict__ lats, uint32_t n_lats, uint32_t sweeps,
+                          uint32_t samples)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_lats) return;
+    lats[i].state[0] += sweeps;
+    lats[i].state[1] += samples;
+}
+
+// ---------------------------------------------------------------- launchers
+static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
+
+hipError_t launch_int8_init(const HbmLattice *d, int n_lats, int L, hipStream_t s)
+{
+    const uint32_t n = (uint32_t)L * (uint32_t)L;
+    hipLaunchKernelGGL(k_int8_init, dim3(cdiv(n, 256u), (uint32_t)n_lats), dim3(256), 0, s, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_int8_half_sweep(const HbmLattice *d, int n_lats, int L, uint32_t t_off,
+                                  uint32_t colour, hipStream_t s)
+{
+    const uint32_t n_sites = (uint32_t)L * (uint32_t)L / 2u;
+    const uint32_t n_words = cdiv(n_sites, 32u);
+    hipLaunchKernelGGL(k_int8_half_sweep, dim3(cdiv(n_words, 256u), (uint32_t)n_lats), dim3(256), 0,
+                       s, d, t_off, colour);
+    return hipGetLastError();
+}
+
+hipError_t launch_int8_measure0(const HbmLattice *d, int n_lats, int L, uint32_t t_off,
+                                hipStream_t s)
+{
+    const uint32_t Lb = (uint32_t)L / 2u;
+    const uint32_t n_items = L >= 64 ? Lb * ((uint32_t)L / 64u) : cdiv(Lb * Lb, 32u);
+    hipLaunchKernelGGL(k_int8_measure0, dim3(cdiv(n_items, 256u), (uint32_t)n_lats), dim3(256), 0,
+                       s, d, t_off);
+    return hipGetLastError();
+}
+
+hipError_t launch_levels_accumulate(const HbmLattice *d, int n_lats, uint32_t t_off,
+                                    uint32_t q_off, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_levels_accumulate, dim3((uint32_t)n_lats), dim3(1024), 0, s, d, t_off,
+                       q_off);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance(const HbmLattice *d, int n_lats, uint32_t sweeps, uint32_t samples,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(k_advance, dim3(cdiv((uint32_t)n_lats, 64u)), dim3(64), 0, s, d,
+                       (uint32_t)n_lats, sweeps, samples);
+    return hipGetLastError();
+}
+
+}  // namespace mcrg

@@ -1,0 +1,62 @@
+// Descriptor and record layout shared by the host runtime and the HBM-schedule
+// kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+
+namespace mcrg {
+
+constexpr uint32_t kMaxLevels = 15;
+// LDS for levels >= 2 of the HBM schedule: level 2 of an L = 2048 lattice is
+// 512 x 512 bits = 8192 words.
+constexpr uint32_t kLdsLevelWords = 8192;
+constexpr int kMaxHbmMeasureL = 2048;
+
+// Per-bin record of one chain measured on levels 0..nl (the reference keeps the
+// same sums as FP64, src/mcrg.cpp:59-66, with 3x3 blocks flattened column-major
+// as src/definitions.cpp:9-19):
+//   S[l][a]                 sum of operator a at level l           3 (nl+1)
+//   SELF[l][a][b]           sum of S[l][a] S[l][b]                 9 (nl+1)
+//   CROSS[l][a][b], l >= 1  sum of S[l][a] S[l-1][b]               9 nl
+//   M, |M|, M^2 at level 0, sample count                          4
+__host__ __device__ constexpr uint32_t off_S(uint32_t l, uint32_t a) { return l * 3u + a; }
+__host__ __device__ constexpr uint32_t off_self(uint32_t nl, uint32_t l, uint32_t a, uint32_t b)
+{
+    return 3u * (nl + 1u) + l * 9u + b * 3u + a;
+}
+__host__ __device__ constexpr uint32_t off_cross(uint32_t nl, uint32_t l, uint32_t a, uint32_t b)
+{
+    return 12u * (nl + 1u) + (l - 1u) * 9u + b * 3u + a;
+}
+__host__ __device__ constexpr uint32_t off_M(uint32_t nl) { return 12u * (nl + 1u) + 9u * nl; }
+__host__ __device__ constexpr uint32_t record_size(uint32_t nl) { return off_M(nl) + 4u; }
+
+struct HbmLattice {
+    int8_t *spins;              // L*L int8, row-major (x fastest)
+    uint32_t *lvl1;             // level-1 bit-packed lattice, (L/2)^2 bits
+    unsigned long long *sums0;  // level-0 disagreement counts dNN, dNNN, dP, ndown
+    int64_t *records;           // max_bins * record_size(n_levels)
+    uint32_t *state;            // [0] sweep counter t, [1] measured-sample counter
+    uint32_t k0, k1;            // Philox key = seed
+    Accept acc;
+    int L;
+    int n_levels;
+    int sweeps_per_bin;
+    int pad;
+};
+
+// Launchers (host).  `d` points to n_lats device descriptors that all share side L.
+hipError_t launch_int8_init(const HbmLattice *d, int n_lats, int L, hipStream_t s);
+hipError_t launch_int8_half_sweep(const HbmLattice *d, int n_lats, int L, uint32_t t_off,
+                                  uint32_t colour, hipStream_t s);
+hipError_t launch_int8_measure0(const HbmLattice *d, int n_lats, int L, uint32_t t_off,
+                                hipStream_t s);
+hipError_t launch_levels_accumulate(const HbmLattice *d, int n_lats, uint32_t t_off,
+                                    uint32_t q_off, hipStream_t s);
+hipError_t launch_advance(const HbmLattice *d, int n_lats, uint32_t sweeps, uint32_t samples,
+                          hipStream_t s);
+
+}  // namespace mcrg

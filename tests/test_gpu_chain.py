@@ -1,0 +1,166 @@
+"""GPU parity of the plain (HBM, int8) chain path against the CPU oracle.
+
+Integer/bit work, so the bar is bit-exact: spins after every checked sweep and
+every int64 record entry.  Cases cover the smallest lattices (rows shorter than
+a 32-site word), the word-aligned fast path, both colours' boundary bytes, the
+L = 2048 configuration's first 100 sweeps, chunking and checkpoint/resume.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KC = 0.5 * math.log(1.0 + math.sqrt(2.0))
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import mcrg_amd
+    return mcrg_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(amd):
+    c = amd.Context(0)
+    yield c
+    c.close()
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("L", [4, 8, 16, 32, 64, 128, 256, 2048])
+def test_init_matches_oracle(amd, ctx, orc, L):
+    _say("init", L)
+    with amd.Chain(ctx, L, KC, seed=12345 + L) as ch:
+        assert np.array_equal(ch.spins(), orc.init_spins(L, 12345 + L))
+
+
+@pytest.mark.parametrize("L,J", [(4, KC), (8, KC), (8, 0.0), (16, 0.3), (32, 0.489652),
+                                 (64, KC), (128, 0.4320459), (256, 2.5), (512, KC)])
+def test_sweeps_bit_exact(amd, ctx, orc, L, J):
+    seed = 0xDEADBEEF00000000 + L
+    _say("sweeps", L, J)
+    with amd.Chain(ctx, L, J, seed=seed) as ch:
+        s = orc.init_spins(L, seed)
+        for t in range(12):
+            ch.sweep(1)
+            orc.sweep(L, s, seed, J, t)
+            assert np.array_equal(ch.spins(), s), f"L={L} J={J} diverged at sweep {t}"
+
+
+def test_config2_first_100_sweeps(amd, ctx, orc):
+    """L = 2048, |K| = 0.4406868, seed 7: bit-exact for sweeps 1..100."""
+    L, J, seed = 2048, 0.4406868, 7
+    _say("L=2048 first 100 sweeps")
+    s = orc.init_spins(L, seed)
+    with amd.Chain(ctx, L, J, seed=seed) as ch:
+        assert np.array_equal(ch.spins(), s)
+        done = 0
+        for upto in (1, 10, 100):
+            for t in range(done, upto):
+                orc.sweep(L, s, seed, J, t)
+            ch.sweep(upto - done)
+            done = upto
+            assert np.array_equal(ch.spins(), s), f"diverged by sweep {upto}"
+
+
+@pytest.mark.parametrize("L,nl,J", [(4, 1, KC), (8, 2, KC), (16, 3, 0.42), (32, 4, KC),
+                                    (64, 5, 0.4496804), (128, 6, KC), (256, 5, KC)])
+def test_measurement_records_bit_exact(amd, ctx, orc, L, nl, J):
+    seed = 99 + L
+    n_therm, n_meas, spb = 5, 23, 7
+    n_bins = -(-n_meas // spb)
+    _say("records", L, nl)
+    s_ref, rec_ref = orc.run_chain(L, J, seed, n_therm, n_meas, nl, spb)
+    with amd.Chain(ctx, L, J, seed=seed, n_levels=nl, sweeps_per_bin=spb, max_bins=n_bins) as ch:
+        ch.sweep(n_therm)
+        ch.sweep(n_meas, measure=True)
+        assert np.array_equal(ch.spins(), s_ref)
+        got = ch.records()
+        assert np.array_equal(got, rec_ref), np.argwhere(got != rec_ref)[:10]
+        assert ch.state() == (n_therm + n_meas, n_meas)
+
+
+@pytest.mark.parametrize("L", [4, 16, 64, 512])
+def test_operators_match_oracle(amd, ctx, orc, L):
+    seed = 5 + L
+    with amd.Chain(ctx, L, KC, seed=seed) as ch:
+        ch.sweep(3)
+        s = ch.spins()
+        ops = ch.operators()
+        assert list(ops[:3]) == list(orc.ops(L, s))
+        assert ops[3] == int(s.astype(np.int64).sum())
+
+
+def test_chunking_is_invisible(amd, ctx):
+    L, seed = 128, 31
+    with amd.Chain(ctx, L, KC, seed=seed, n_levels=4, sweeps_per_bin=5, max_bins=4) as a, \
+            amd.Chain(ctx, L, KC, seed=seed, n_levels=4, sweeps_per_bin=5, max_bins=4) as b:
+        a.sweep(6)
+        a.sweep(20, measure=True)
+        for _ in range(6):
+            b.sweep(1)
+        for _ in range(20):
+            b.sweep(1, measure=True)
+        assert np.array_equal(a.spins(), b.spins())
+        assert np.array_equal(a.records(), b.records())
+
+
+def test_checkpoint_resume(amd, ctx):
+    L, seed, nl = 256, 4242, 3
+    with amd.Chain(ctx, L, KC, seed=seed, n_levels=nl, sweeps_per_bin=10, max_bins=3) as a:
+        a.sweep(15)
+        a.sweep(30, measure=True)
+        want_s, want_r = a.spins(), a.records()
+    with amd.Chain(ctx, L, KC, seed=seed, n_levels=nl, sweeps_per_bin=10, max_bins=3) as b:
+        b.sweep(15)
+        b.sweep(12, measure=True)
+        snap = (b.spins(), b.state(), b.records())
+    with amd.Chain(ctx, L, KC, seed=seed, n_levels=nl, sweeps_per_bin=10, max_bins=3) as c:
+        c.set_spins(snap[0])
+        c.set_state(snap[1][0], snap[1][1], snap[2])
+        c.sweep(18, measure=True)
+        assert np.array_equal(c.spins(), want_s)
+        assert np.array_equal(c.records(), want_r)
+
+
+def test_set_coupling_matches_oracle(amd, ctx, orc):
+    L, seed = 64, 8
+    s = orc.init_spins(L, seed)
+    with amd.Chain(ctx, L, 0.3, seed=seed) as ch:
+        ch.sweep(4)
+        for t in range(4):
+            orc.sweep(L, s, seed, 0.3, t)
+        ch.set_coupling(0.5)
+        ch.sweep(4)
+        for t in range(4, 8):
+            orc.sweep(L, s, seed, 0.5, t)
+        assert np.array_equal(ch.spins(), s)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(L=12), "power of two"), (dict(L=2), "power of two"), (dict(coupling=-0.44), "coupling"),
+    (dict(n_levels=7, L=64), "n_levels"), (dict(sweeps_per_bin=0), "sweeps_per_bin"),
+    (dict(L=4096, n_levels=2), "2048"),
+])
+def test_invalid_chain_arguments(amd, ctx, kw, msg):
+    args = dict(L=64, coupling=KC, seed=1, n_levels=1)
+    args.update(kw)
+    with pytest.raises(amd.MCRGError) as e:
+        amd.Chain(ctx, **args)
+    assert e.value.status == amd.ERR_INVALID_ARG
+    assert msg in str(e.value)
+
+
+def test_record_capacity_error(amd, ctx):
+    with amd.Chain(ctx, 16, KC, seed=1, n_levels=1, sweeps_per_bin=4, max_bins=2) as ch:
+        ch.sweep(8, measure=True)
+        with pytest.raises(amd.MCRGError) as e:
+            ch.sweep(1, measure=True)
+        assert e.value.status == amd.ERR_CAPACITY
