@@ -69,6 +69,18 @@ int32_t mcrg_record_size(int32_t n_levels);
 mcrg_status mcrg_ctx_create(int32_t device, mcrg_ctx *out);
 mcrg_status mcrg_ctx_destroy(mcrg_ctx ctx);
 
+/* Properties of the context's device (no reference counterpart: the reference
+ * runs on host cores only). */
+typedef struct {
+    char name[64];            /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+    int32_t compute_units;
+    int32_t clock_khz;
+    int32_t lds_per_block;    /* bytes a workgroup may allocate            */
+    int32_t warp_size;
+    int64_t global_mem_bytes;
+} mcrg_device_info;
+mcrg_status mcrg_ctx_device_info(mcrg_ctx ctx, mcrg_device_info *out);
+
 /* ------------------------------------------------------------------ chains */
 typedef struct {
     int32_t L;              /* side, power of two, 4 <= L <= 16384          */
@@ -121,6 +133,49 @@ mcrg_status mcrg_chain_set_state(mcrg_chain ch, uint32_t sweep, uint32_t n_measu
  * 2*out[0], 2*out[1]; IsingModel::calc_magnetization (src/ising.cpp:176-179)
  * is out[3] / L^2. */
 mcrg_status mcrg_chain_operators(mcrg_chain ch, int64_t out[4]);
+
+/* ------------------------------------------------------------------ batches */
+/* Many independent chains (uneven L and coupling) advanced together by the
+ * LDS-resident schedule: bit-packed lattices held in LDS by a persistent grid,
+ * one launch per chunk of sweeps.  A job is the reference's whole
+ * calc_critical_exponent chain (src/mcrg.cpp:22-98) for one (L, K, seed):
+ * n_therm unmeasured sweeps (equilibrate, src/ising.cpp:14-84) then n_meas
+ * measured sweeps.  Every job is bit-identical to the same chain run alone
+ * through mcrg_chain_* (same seed, same Philox counters).  The reference
+ * spreads chains over MPI ranks (split_samples, src/definitions.cpp:79-87);
+ * here they share one device. */
+typedef struct mcrg_batch_s *mcrg_batch;
+
+typedef struct {
+    int32_t L;              /* power of two, 8 <= L <= 512                    */
+    double coupling;        /* J = |K| >= 0                                    */
+    uint64_t seed;          /* Philox key                                      */
+    int32_t n_therm;        /* unmeasured sweeps                               */
+    int32_t n_meas;         /* measured sweeps                                 */
+    int32_t n_levels;       /* blockings per sample, <= log2(L) - 1            */
+    int32_t sweeps_per_bin; /* measured sweeps per record bin (>= 1); the job
+                               has ceil(n_meas / sweeps_per_bin) bins         */
+} mcrg_job_desc;
+
+typedef struct {
+    int32_t threads;           /* workgroup size 64/128/256/512; 0 = 256       */
+    int32_t sweeps_per_launch; /* sweeps per kernel launch; 0 = 250           */
+    int32_t max_group;         /* upper bound on lattices per workgroup; 0 = 256 */
+} mcrg_batch_opts;
+
+mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n_jobs,
+                              const mcrg_batch_opts *opts, mcrg_batch *out);
+mcrg_status mcrg_batch_destroy(mcrg_batch b);
+/* Enqueues launches that advance every unfinished job by up to max_sweeps
+ * sweeps (asynchronous).  *remaining receives the largest number of sweeps any
+ * job still has to run after them. */
+mcrg_status mcrg_batch_run(mcrg_batch b, int32_t max_sweeps, int32_t *remaining);
+mcrg_status mcrg_batch_sync(mcrg_batch b);
+/* Records of job j: n = ceil(n_meas / sweeps_per_bin) * mcrg_record_size(n_levels). */
+mcrg_status mcrg_batch_get_records(mcrg_batch b, int32_t job, int64_t *host, int64_t n);
+/* Spins of job j as +-1 int8, row-major, n = L*L. */
+mcrg_status mcrg_batch_get_spins(mcrg_batch b, int32_t job, int8_t *host, int64_t n);
+mcrg_status mcrg_batch_get_state(mcrg_batch b, int32_t job, uint32_t *sweep, uint32_t *n_measured);
 
 /* ------------------------------------------------------------------ timing */
 /* Kernel-duration statistics measured with HIP events on the stream that

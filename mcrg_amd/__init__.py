@@ -41,6 +41,23 @@ class ChainDesc(ctypes.Structure):
                 ("sweeps_per_bin", ctypes.c_int32), ("max_bins", ctypes.c_int32)]
 
 
+class JobDesc(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_int32), ("coupling", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("n_therm", ctypes.c_int32), ("n_meas", ctypes.c_int32), ("n_levels", ctypes.c_int32),
+                ("sweeps_per_bin", ctypes.c_int32)]
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_int32), ("sweeps_per_launch", ctypes.c_int32),
+                ("max_group", ctypes.c_int32)]
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("compute_units", ctypes.c_int32), ("clock_khz", ctypes.c_int32),
+                ("lds_per_block", ctypes.c_int32), ("warp_size", ctypes.c_int32),
+                ("global_mem_bytes", ctypes.c_int64)]
+
+
 _c = ctypes
 _p = ctypes.c_void_p
 _SIGS = {
@@ -51,6 +68,7 @@ _SIGS = {
     "mcrg_ctx_create": (_c.c_int32, [_c.c_int32, _c.POINTER(_p)]),
     "mcrg_ctx_destroy": (_c.c_int32, [_p]),
     "mcrg_ctx_set_timing": (_c.c_int32, [_p, _c.c_int32]),
+    "mcrg_ctx_device_info": (_c.c_int32, [_p, _c.POINTER(DeviceInfo)]),
     "mcrg_ctx_kernel_time": (_c.c_int32, [_p, _c.c_int32, _c.POINTER(_c.c_int64), _c.POINTER(_c.c_double)]),
     "mcrg_chain_create": (_c.c_int32, [_p, _c.POINTER(ChainDesc), _c.POINTER(_p)]),
     "mcrg_chain_destroy": (_c.c_int32, [_p]),
@@ -63,6 +81,14 @@ _SIGS = {
     "mcrg_chain_get_state": (_c.c_int32, [_p, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint32)]),
     "mcrg_chain_set_state": (_c.c_int32, [_p, _c.c_uint32, _c.c_uint32, _p, _c.c_int64]),
     "mcrg_chain_operators": (_c.c_int32, [_p, _c.POINTER(_c.c_int64)]),
+    "mcrg_batch_create": (_c.c_int32, [_p, _c.POINTER(JobDesc), _c.c_int32, _c.POINTER(BatchOpts),
+                                       _c.POINTER(_p)]),
+    "mcrg_batch_destroy": (_c.c_int32, [_p]),
+    "mcrg_batch_run": (_c.c_int32, [_p, _c.c_int32, _c.POINTER(_c.c_int32)]),
+    "mcrg_batch_sync": (_c.c_int32, [_p]),
+    "mcrg_batch_get_records": (_c.c_int32, [_p, _c.c_int32, _p, _c.c_int64]),
+    "mcrg_batch_get_spins": (_c.c_int32, [_p, _c.c_int32, _p, _c.c_int64]),
+    "mcrg_batch_get_state": (_c.c_int32, [_p, _c.c_int32, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint32)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(_lib, _name)
@@ -109,6 +135,13 @@ class Context:
         h = ctypes.c_void_p()
         check(_lib.mcrg_ctx_create(device, ctypes.byref(h)))
         self.handle = h
+
+    def device_info(self):
+        d = DeviceInfo()
+        check(_lib.mcrg_ctx_device_info(self.handle, ctypes.byref(d)))
+        return {"name": d.name.decode(), "compute_units": d.compute_units, "clock_khz": d.clock_khz,
+                "lds_per_block": d.lds_per_block, "warp_size": d.warp_size,
+                "global_mem_bytes": d.global_mem_bytes}
 
     def set_timing(self, enable=True):
         check(_lib.mcrg_ctx_set_timing(self.handle, 1 if enable else 0))
@@ -186,6 +219,69 @@ class Chain:
     def close(self):
         if self.handle:
             _lib.mcrg_chain_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def n_bins(n_meas, sweeps_per_bin):
+    return max(1, -(-n_meas // sweeps_per_bin))
+
+
+class Batch:
+    """Independent chains advanced together on the LDS-resident schedule
+    (mcrg_batch).  `jobs` is a list of dicts with keys L, coupling, seed,
+    n_therm, n_meas, n_levels, sweeps_per_bin."""
+
+    def __init__(self, ctx, jobs, threads=0, sweeps_per_launch=0, max_group=0):
+        arr = (JobDesc * len(jobs))()
+        for i, j in enumerate(jobs):
+            arr[i] = JobDesc(j["L"], float(j["coupling"]), j["seed"], j.get("n_therm", 0), j.get("n_meas", 0),
+                             j.get("n_levels", 0), j.get("sweeps_per_bin", 1))
+        opts = BatchOpts(threads, sweeps_per_launch, max_group)
+        h = ctypes.c_void_p()
+        check(_lib.mcrg_batch_create(ctx.handle, arr, len(jobs), ctypes.byref(opts), ctypes.byref(h)))
+        self.handle = h
+        self.jobs = [dict(j) for j in jobs]
+
+    def run(self, max_sweeps):
+        rem = ctypes.c_int32()
+        check(_lib.mcrg_batch_run(self.handle, max_sweeps, ctypes.byref(rem)))
+        return rem.value
+
+    def run_all(self):
+        while self.run(1 << 30) > 0:
+            pass
+
+    def sync(self):
+        check(_lib.mcrg_batch_sync(self.handle))
+
+    def records(self, j):
+        job = self.jobs[j]
+        rs = record_size(job.get("n_levels", 0))
+        nb = n_bins(job.get("n_meas", 0), job.get("sweeps_per_bin", 1))
+        a = np.zeros(nb * rs, dtype=np.int64)
+        check(_lib.mcrg_batch_get_records(self.handle, j, _ptr(a), a.size))
+        return a.reshape(nb, rs)
+
+    def spins(self, j):
+        L = self.jobs[j]["L"]
+        a = np.zeros(L * L, dtype=np.int8)
+        check(_lib.mcrg_batch_get_spins(self.handle, j, _ptr(a), a.size))
+        return a
+
+    def state(self, j):
+        t, q = ctypes.c_uint32(), ctypes.c_uint32()
+        check(_lib.mcrg_batch_get_state(self.handle, j, ctypes.byref(t), ctypes.byref(q)))
+        return t.value, q.value
+
+    def close(self):
+        if self.handle:
+            _lib.mcrg_batch_destroy(self.handle)
             self.handle = None
 
     def __enter__(self):

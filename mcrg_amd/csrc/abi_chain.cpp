@@ -142,6 +142,21 @@ mcrg_status mcrg_ctx_destroy(mcrg_ctx ctx)
     return MCRG_OK;
 }
 
+mcrg_status mcrg_ctx_device_info(mcrg_ctx ctx, mcrg_device_info *out)
+{
+    MCRG_REQUIRE(ctx != nullptr && out != nullptr, "null argument");
+    hipDeviceProp_t p;
+    MCRG_HIP(hipGetDeviceProperties(&p, ctx->device));
+    std::memset(out, 0, sizeof(*out));
+    std::strncpy(out->name, p.gcnArchName, sizeof(out->name) - 1);
+    out->compute_units = p.multiProcessorCount;
+    out->clock_khz = p.clockRate;
+    out->lds_per_block = (int32_t)p.sharedMemPerBlock;
+    out->warp_size = p.warpSize;
+    out->global_mem_bytes = (int64_t)p.totalGlobalMem;
+    return MCRG_OK;
+}
+
 mcrg_status mcrg_ctx_set_timing(mcrg_ctx ctx, int32_t enable)
 {
     MCRG_REQUIRE(ctx != nullptr, "ctx is null");

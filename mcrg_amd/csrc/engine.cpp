@@ -1,0 +1,409 @@
+// Host engine of the LDS-resident schedule.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+#include "runtime.hpp"
+
+namespace mcrg {
+
+static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// ------------------------------------------------------------------ Arena
+hipError_t Arena::init(size_t bytes)
+{
+    cap_ = round_up(bytes, 256);
+    used_ = 0;
+    free_.clear();
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, cap_);
+    if (e != hipSuccess) {
+        cap_ = 0;
+        return e;
+    }
+    base_ = static_cast<char *>(p);
+    free_[0] = cap_;
+    return hipSuccess;
+}
+
+void Arena::release()
+{
+    if (base_) (void)hipFree(base_);
+    base_ = nullptr;
+    cap_ = used_ = 0;
+    free_.clear();
+}
+
+bool Arena::alloc(size_t bytes, size_t &off)
+{
+    bytes = round_up(bytes, 256);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second >= bytes) {
+            off = it->first;
+            const size_t rest = it->second - bytes;
+            free_.erase(it);
+            if (rest) free_[off + bytes] = rest;
+            used_ += bytes;
+            return true;
+        }
+    }
+    return false;
+}
+
+void Arena::free(size_t off, size_t bytes)
+{
+    bytes = round_up(bytes, 256);
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            free_.erase(it);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ engine
+ResidentEngine::~ResidentEngine()
+{
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    arena_.release();
+    if (d_lats_) (void)hipFree(d_lats_);
+    if (d_slots_) (void)hipFree(d_slots_);
+    if (d_slot_lats_) (void)hipFree(d_slot_lats_);
+    if (d_queue_) (void)hipFree(d_queue_);
+    if (d_init_idx_) (void)hipFree(d_init_idx_);
+}
+
+hipError_t ResidentEngine::init(int device, size_t arena_bytes, int capacity, const EngineOptions &opt,
+                                hipStream_t s)
+{
+    device_ = device;
+    stream_ = s;
+    opt_ = opt;
+    capacity_ = capacity;
+    hipError_t e;
+    if ((e = arena_.init(arena_bytes)) != hipSuccess) return e;
+    if ((e = hipMalloc(&d_lats_, sizeof(PackedLattice) * (size_t)capacity)) != hipSuccess) return e;
+    if ((e = hipMalloc(&d_queue_, 256)) != hipSuccess) return e;
+    if ((e = hipMalloc(&d_init_idx_, sizeof(int32_t) * (size_t)capacity)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+std::string ResidentEngine::check(const JobSpec &j)
+{
+    if (!is_pow2(j.L) || j.L < kResidentMinL || j.L > kResidentMaxL)
+        return "L must be a power of two in [8, 512] for the LDS-resident schedule";
+    if (!(j.coupling >= 0.0) || !std::isfinite(j.coupling)) return "coupling J = |K| must be >= 0";
+    if (j.n_levels < 0 || j.n_levels > ilog2(j.L) - 1) return "n_levels must be in [0, log2(L) - 1]";
+    if (j.sweeps_per_bin < 1) return "sweeps_per_bin must be >= 1";
+    if ((uint64_t)j.n_therm + (uint64_t)j.n_meas > 0xFFFFFFFFull) return "too many sweeps";
+    return std::string();
+}
+
+static int bins_of(const JobSpec &j)
+{
+    const int b = (int)((j.n_meas + (uint32_t)j.sweeps_per_bin - 1) / (uint32_t)j.sweeps_per_bin);
+    return b > 0 ? b : 1;
+}
+
+size_t ResidentEngine::job_bytes(const JobSpec &j)
+{
+    const size_t spins = round_up(2u * cb_words((uint32_t)j.L) * 4u, 256);
+    const size_t rec = round_up((size_t)bins_of(j) * record_size((uint32_t)j.n_levels) * 8u, 256);
+    return spins + rec;
+}
+
+uint32_t ResidentEngine::lds_bytes(int L, int G, int nl) const
+{
+    return slot_layout((uint32_t)L, (uint32_t)G, (uint32_t)nl).total * 4u;
+}
+
+int ResidentEngine::group_size(int L, int nl) const
+{
+    const int W = (int)cb_words((uint32_t)L);
+    int G = W >= opt_.threads ? 1 : opt_.threads / W;
+    G = std::min(G, opt_.max_group);
+    // keep a slot's LDS modest so that several workgroups share a CU
+    const uint32_t soft = 64u * 1024u;
+    while (G > 1 && lds_bytes(L, G, nl) > soft) G /= 2;
+    return std::max(G, 1);
+}
+
+int ResidentEngine::add(const JobSpec &j)
+{
+    int id;
+    if (!free_ids_.empty()) {
+        id = free_ids_.back();
+    } else {
+        if ((int)jobs_.size() >= capacity_) return -1;
+        id = (int)jobs_.size();
+    }
+    const size_t bytes = job_bytes(j);
+    size_t off = 0;
+    if (!arena_.alloc(bytes, off)) return -1;
+    if (!free_ids_.empty()) free_ids_.pop_back();
+    else jobs_.emplace_back();
+    Job &job = jobs_[(size_t)id];
+    job = Job();
+    job.spec = j;
+    job.live = true;
+    job.off = off;
+    job.bytes = bytes;
+    job.n_bins = bins_of(j);
+    job.therm_left = j.n_therm;
+    job.meas_left = j.n_meas;
+    pending_init_.push_back(id);
+    slots_dirty_ = true;
+    return id;
+}
+
+void ResidentEngine::remove(int id)
+{
+    if (!active(id)) return;
+    Job &job = jobs_[(size_t)id];
+    arena_.free(job.off, job.bytes);
+    job.live = false;
+    free_ids_.push_back(id);
+    pending_init_.erase(std::remove(pending_init_.begin(), pending_init_.end(), id), pending_init_.end());
+    slots_dirty_ = true;
+}
+
+bool ResidentEngine::finished(int id) const
+{
+    const Job &j = jobs_[(size_t)id];
+    return j.therm_left + j.meas_left == 0;
+}
+
+uint32_t ResidentEngine::remaining(int id) const
+{
+    const Job &j = jobs_[(size_t)id];
+    return j.therm_left + j.meas_left;
+}
+
+uint32_t ResidentEngine::max_remaining() const
+{
+    uint32_t m = 0;
+    for (const Job &j : jobs_)
+        if (j.live) m = std::max(m, j.therm_left + j.meas_left);
+    return m;
+}
+
+int ResidentEngine::n_live() const
+{
+    int n = 0;
+    for (const Job &j : jobs_) n += j.live ? 1 : 0;
+    return n;
+}
+
+void ResidentEngine::state(int id, uint32_t &t, uint32_t &n_measured) const
+{
+    t = jobs_[(size_t)id].t;
+    n_measured = jobs_[(size_t)id].n_measured;
+}
+
+hipError_t ResidentEngine::flush_pending()
+{
+    if (pending_init_.empty()) return hipSuccess;
+    hipError_t e;
+    // descriptors of the new lattices, then one init launch for all of them
+    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
+    staging_lats_.resize(pending_init_.size());
+    for (size_t k = 0; k < pending_init_.size(); ++k) {
+        const int id = pending_init_[k];
+        const Job &job = jobs_[(size_t)id];
+        PackedLattice &P = staging_lats_[k];
+        std::memset(&P, 0, sizeof(P));
+        const uint32_t W = cb_words((uint32_t)job.spec.L);
+        P.spins = reinterpret_cast<uint32_t *>(arena_.base() + job.off);
+        P.records = reinterpret_cast<int64_t *>(arena_.base() + job.off + round_up(2u * W * 4u, 256));
+        P.k0 = (uint32_t)job.spec.seed;
+        P.k1 = (uint32_t)(job.spec.seed >> 32);
+        P.acc = make_accept(job.spec.coupling);
+        P.t = 0;
+        P.n_measured = 0;
+        P.therm_left = job.spec.n_therm;
+        P.meas_left = job.spec.n_meas;
+        P.L = job.spec.L;
+        P.n_levels = job.spec.n_levels;
+        P.sweeps_per_bin = job.spec.sweeps_per_bin;
+        P.max_bins = job.n_bins;
+        if ((e = hipMemcpyAsync(d_lats_ + id, &P, sizeof(P), hipMemcpyHostToDevice, stream_)) != hipSuccess)
+            return e;
+        const size_t rec_bytes = (size_t)job.n_bins * record_size((uint32_t)job.spec.n_levels) * 8u;
+        if ((e = hipMemsetAsync(P.records, 0, rec_bytes, stream_)) != hipSuccess) return e;
+    }
+    if ((e = hipMemcpyAsync(d_init_idx_, pending_init_.data(), sizeof(int32_t) * pending_init_.size(),
+                            hipMemcpyHostToDevice, stream_)) != hipSuccess)
+        return e;
+    if ((e = launch_packed_init(d_lats_, d_init_idx_, (int)pending_init_.size(), stream_)) != hipSuccess)
+        return e;
+    // the copies above read host memory that is reused later
+    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
+    pending_init_.clear();
+    return hipSuccess;
+}
+
+void ResidentEngine::rebuild_slots(uint32_t chunk)
+{
+    struct Key {
+        int L, nl;
+        bool operator<(const Key &o) const { return L != o.L ? L > o.L : nl < o.nl; }
+    };
+    std::map<Key, std::vector<int>> groups;
+    for (size_t id = 0; id < jobs_.size(); ++id) {
+        const Job &j = jobs_[id];
+        if (j.live && j.therm_left + j.meas_left > 0) groups[{j.spec.L, j.spec.n_levels}].push_back((int)id);
+    }
+    struct Pending {
+        Slot s;
+        uint64_t work;
+    };
+    std::vector<Pending> pend;
+    slot_lats_.clear();
+    max_lds_ = 0;
+    for (auto &kv : groups) {
+        const int L = kv.first.L, nl = kv.first.nl;
+        const int G = group_size(L, nl);
+        const std::vector<int> &ids = kv.second;
+        for (size_t a = 0; a < ids.size(); a += (size_t)G) {
+            const int g = (int)std::min<size_t>((size_t)G, ids.size() - a);
+            Slot s{L, g, nl, (int32_t)slot_lats_.size()};
+            uint64_t work = 0;
+            for (int k = 0; k < g; ++k) {
+                const int id = ids[a + (size_t)k];
+                slot_lats_.push_back(id);
+                const Job &j = jobs_[(size_t)id];
+                work += (uint64_t)L * (uint64_t)L * std::min<uint64_t>(chunk, j.therm_left + j.meas_left);
+            }
+            pend.push_back({s, work});
+            max_lds_ = std::max(max_lds_, lds_bytes(L, g, nl));
+        }
+    }
+    // longest slots first: the persistent grid pulls them in this order
+    std::stable_sort(pend.begin(), pend.end(), [](const Pending &a, const Pending &b) { return a.work > b.work; });
+    slots_.clear();
+    for (auto &p : pend) slots_.push_back(p.s);
+}
+
+hipError_t ResidentEngine::upload_slots()
+{
+    hipError_t e;
+    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
+    if (slots_.size() > slot_cap_) {
+        if (d_slots_) (void)hipFree(d_slots_);
+        slot_cap_ = std::max<size_t>(slots_.size(), 2 * slot_cap_);
+        if ((e = hipMalloc(&d_slots_, sizeof(Slot) * slot_cap_)) != hipSuccess) return e;
+    }
+    if (slot_lats_.size() > slot_lat_cap_) {
+        if (d_slot_lats_) (void)hipFree(d_slot_lats_);
+        slot_lat_cap_ = std::max<size_t>(slot_lats_.size(), 2 * slot_lat_cap_);
+        if ((e = hipMalloc(&d_slot_lats_, sizeof(int32_t) * slot_lat_cap_)) != hipSuccess) return e;
+    }
+    if (!slots_.empty()) {
+        if ((e = hipMemcpy(d_slots_, slots_.data(), sizeof(Slot) * slots_.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+            return e;
+        if ((e = hipMemcpy(d_slot_lats_, slot_lats_.data(), sizeof(int32_t) * slot_lats_.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return e;
+    }
+    grid_ = resident_max_active_groups(opt_.threads, max_lds_);
+    if (grid_ <= 0) {
+        failed_ = "hipOccupancyMaxActiveBlocksPerMultiprocessor";
+        return hipErrorInvalidConfiguration;
+    }
+    grid_ = std::min<int>(grid_, (int)slots_.size());
+    return hipSuccess;
+}
+
+hipError_t ResidentEngine::tick(uint32_t n, uint32_t &advanced, KernelTimer *tm)
+{
+    advanced = 0;
+    failed_ = "";
+    hipError_t e;
+    if ((e = flush_pending()) != hipSuccess) {
+        failed_ = "flush_pending";
+        return e;
+    }
+    const uint32_t chunk = std::min<uint32_t>(n, (uint32_t)opt_.sweeps_per_launch);
+    if (chunk == 0 || max_remaining() == 0) return hipSuccess;
+    if (slots_dirty_ || chunk != slots_chunk_) {
+        rebuild_slots(chunk);
+        if ((e = upload_slots()) != hipSuccess) {
+            if (!*failed_) failed_ = "upload_slots";
+            return e;
+        }
+        slots_dirty_ = false;
+        slots_chunk_ = chunk;
+    }
+    if (slots_.empty()) return hipSuccess;
+    const int ev = tm ? tm->begin(kKindResident, stream_) : -1;
+    e = launch_resident(opt_.threads, d_slots_, (uint32_t)slots_.size(), d_slot_lats_, d_lats_, d_queue_, chunk,
+                        max_lds_, (uint32_t)grid_, stream_);
+    if (tm) tm->end(ev, stream_);
+    if (e != hipSuccess) {
+        failed_ = "launch_resident";
+        return e;
+    }
+    // mirror the device counters
+    bool any_done = false;
+    for (Job &j : jobs_) {
+        if (!j.live) continue;
+        const uint32_t total = j.therm_left + j.meas_left;
+        if (total == 0) continue;
+        const uint32_t swept = std::min(total, chunk);
+        const uint32_t therm = std::min(swept, j.therm_left);
+        const uint32_t meas = swept - therm;
+        j.t += swept;
+        j.therm_left -= therm;
+        j.meas_left -= meas;
+        j.n_measured += meas;
+        if (j.therm_left + j.meas_left == 0) any_done = true;
+    }
+    if (any_done) slots_dirty_ = true;
+    advanced = chunk;
+    return hipSuccess;
+}
+
+hipError_t ResidentEngine::read_spins(int id, int8_t *host)
+{
+    hipError_t e;
+    if ((e = flush_pending()) != hipSuccess) return e;
+    const Job &job = jobs_[(size_t)id];
+    const uint32_t L = (uint32_t)job.spec.L, W = cb_words(L), half = L / 2;
+    std::vector<uint32_t> words(2u * W);
+    if ((e = hipMemcpyAsync(words.data(), arena_.base() + job.off, words.size() * 4u, hipMemcpyDeviceToHost,
+                            stream_)) != hipSuccess)
+        return e;
+    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
+    for (uint32_t c = 0; c < 2u; ++c)
+        for (uint32_t w = 0; w < W; ++w)
+            for (uint32_t b = 0; b < 32u; ++b) {
+                const uint32_t mc = 32u * w + b;
+                const uint32_t y = mc / half, k = mc % half;
+                const uint32_t x = 2u * k + ((y + c) & 1u);
+                host[y * L + x] = ((words[c * W + w] >> b) & 1u) ? -1 : 1;
+            }
+    return hipSuccess;
+}
+
+hipError_t ResidentEngine::read_records(int id, int64_t *host, int64_t n)
+{
+    hipError_t e;
+    if ((e = flush_pending()) != hipSuccess) return e;
+    const Job &job = jobs_[(size_t)id];
+    const uint32_t W = cb_words((uint32_t)job.spec.L);
+    const char *rec = arena_.base() + job.off + round_up(2u * W * 4u, 256);
+    if ((e = hipMemcpyAsync(host, rec, (size_t)n * 8u, hipMemcpyDeviceToHost, stream_)) != hipSuccess) return e;
+    return hipStreamSynchronize(stream_);
+}
+
+}  // namespace mcrg

@@ -1,0 +1,127 @@
+// Host engine for the LDS-resident schedule: owns the lattices of many jobs in
+// one device arena, groups the active ones into slots and launches chunks.
+// Used by the batch and server entry points of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "resident.hpp"
+
+namespace mcrg {
+
+struct JobSpec {
+    int L = 0;
+    double coupling = 0.0;
+    uint64_t seed = 0;
+    uint32_t n_therm = 0, n_meas = 0;
+    int n_levels = 0;
+    int sweeps_per_bin = 1;
+};
+
+// First-fit allocator over one device allocation (offsets in bytes).
+class Arena {
+  public:
+    hipError_t init(size_t bytes);
+    void release();
+    // Returns false when no free range is large enough.
+    bool alloc(size_t bytes, size_t &off);
+    void free(size_t off, size_t bytes);
+    char *base() const { return base_; }
+    size_t capacity() const { return cap_; }
+    size_t in_use() const { return used_; }
+
+  private:
+    char *base_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
+    std::map<size_t, size_t> free_;  // offset -> size
+};
+
+struct EngineOptions {
+    int threads = 256;           // workgroup size of the resident kernel
+    int sweeps_per_launch = 250;  // chunk length
+    int max_group = 256;          // upper bound on lattices per slot
+};
+
+class ResidentEngine {
+  public:
+    ~ResidentEngine();
+    hipError_t init(int device, size_t arena_bytes, int capacity, const EngineOptions &opt, hipStream_t s);
+
+    // Validates a job for this engine; empty string when it is admissible.
+    static std::string check(const JobSpec &j);
+    // Device bytes a job needs (spins + records), rounded as the arena rounds.
+    static size_t job_bytes(const JobSpec &j);
+    // Allocates and initialises the lattice on the stream; returns its id, or
+    // -1 when the arena or the descriptor table is full.
+    int add(const JobSpec &j);
+    void remove(int id);
+
+    // Enqueues one chunk of min(n, sweeps_per_launch) sweeps for every active
+    // lattice; `advanced` is the chunk length (0 when nothing is left).
+    hipError_t tick(uint32_t n, uint32_t &advanced, struct KernelTimer *tm);
+    // Uploads descriptors and draws the start of lattices added since the last call.
+    hipError_t flush_pending();
+
+    bool active(int id) const { return id >= 0 && id < (int)jobs_.size() && jobs_[id].live; }
+    bool finished(int id) const;
+    uint32_t remaining(int id) const;
+    uint32_t max_remaining() const;
+    int n_live() const;
+
+    hipError_t read_spins(int id, int8_t *host);  // row-major +-1 (synchronous)
+    hipError_t read_records(int id, int64_t *host, int64_t n);
+    void state(int id, uint32_t &t, uint32_t &n_measured) const;
+    const JobSpec &spec(int id) const { return jobs_[id].spec; }
+    int n_bins(int id) const { return jobs_[id].n_bins; }
+    size_t arena_in_use() const { return arena_.in_use(); }
+    size_t arena_capacity() const { return arena_.capacity(); }
+    hipStream_t stream() const { return stream_; }
+    // Name of the HIP call behind the last error returned by this engine.
+    const char *failed_call() const { return failed_; }
+    int group_size(int L, int nl) const;
+    uint32_t lds_bytes(int L, int G, int nl) const;
+
+  private:
+    struct Job {
+        JobSpec spec;
+        bool live = false;
+        size_t off = 0, bytes = 0;
+        int n_bins = 0;
+        // host mirror of the device counters (exact: chunks advance deterministically)
+        uint32_t t = 0, n_measured = 0, therm_left = 0, meas_left = 0;
+    };
+    void rebuild_slots(uint32_t chunk);
+    hipError_t upload_slots();
+
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    EngineOptions opt_;
+    Arena arena_;
+    std::vector<Job> jobs_;
+    std::vector<int> free_ids_;
+    PackedLattice *d_lats_ = nullptr;
+    int capacity_ = 0;
+    // slot tables: host copies and device mirrors
+    std::vector<Slot> slots_;
+    std::vector<int32_t> slot_lats_;
+    Slot *d_slots_ = nullptr;
+    int32_t *d_slot_lats_ = nullptr;
+    size_t slot_cap_ = 0, slot_lat_cap_ = 0;
+    uint32_t *d_queue_ = nullptr;
+    int32_t *d_init_idx_ = nullptr;
+    bool slots_dirty_ = true;
+    uint32_t slots_chunk_ = 0;
+    std::vector<int32_t> pending_init_;
+    std::vector<PackedLattice> staging_lats_;
+    uint32_t max_lds_ = 0;
+    int grid_ = 0;
+    const char *failed_ = "";
+};
+
+}  // namespace mcrg

@@ -12,6 +12,7 @@
 //                         (src/mcrg.cpp:72-98)
 #include "device_common.hpp"
 #include "hbm_kernels.hpp"
+#include "levels.hpp"
 
 namespace mcrg {
 
@@ -247,50 +248,12 @@ __global__ __launch_bounds__(256) void k_int8_measure0(const HbmLattice *__restr
 }
 
 // ---------------------------------------------------------------- levels >= 1
-// Bit-packed row-major lattice ("RM"): site (x, y) of an Ll x Ll level at bit
-// m = y*Ll + x, word m >> 5.  Rows shorter than 32 bits share words.
-
-// Row y of a level with Ll <= 32 in the low Ll bits.
-__device__ __forceinline__ uint32_t rm_row_small(const uint32_t *buf, uint32_t Ll, uint32_t y)
-{
-    const uint32_t m = y * Ll;
-    const uint32_t mask = Ll >= 32u ? 0xFFFFFFFFu : ((1u << Ll) - 1u);
-    return (buf[m >> 5] >> (m & 31u)) & mask;
-}
-
-__device__ __forceinline__ uint32_t rot_right_small(uint32_t v, uint32_t Ll)
-{
-    // bit x of the result = bit x+1 (mod Ll) of v: the "right neighbour" row.
-    return ((v >> 1) | ((v & 1u) << (Ll - 1u))) & ((1u << Ll) - 1u);
-}
-
-// Disagreement counts of one level, summed over the workgroup into cnt[0..2].
+// Disagreement counts of one RM level, summed over the workgroup into cnt[0..2].
 __device__ void rm_ops(const uint32_t *buf, uint32_t Ll, uint32_t *cnt)
 {
     uint32_t dnn = 0, dnnn = 0, dp = 0;
-    if (Ll >= 32u) {
-        const uint32_t wpr = Ll >> 5;
-        const uint32_t n = Ll * wpr;
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-            const uint32_t y = i / wpr, q = i % wpr;
-            const uint32_t yn = (y + 1u) & (Ll - 1u), qn = (q + 1u) % wpr;
-            const uint32_t a = buf[y * wpr + q], an = buf[y * wpr + qn];
-            const uint32_t d = buf[yn * wpr + q], dn = buf[yn * wpr + qn];
-            const uint32_t r = (a >> 1) | (an << 31), g = (d >> 1) | (dn << 31);
-            dnn += popc(a ^ r) + popc(a ^ d);
-            dnnn += popc(a ^ g) + popc(r ^ d);
-            dp += popc(a ^ r ^ d ^ g);
-        }
-    } else {
-        for (uint32_t y = threadIdx.x; y < Ll; y += blockDim.x) {
-            const uint32_t a = rm_row_small(buf, Ll, y);
-            const uint32_t d = rm_row_small(buf, Ll, (y + 1u) & (Ll - 1u));
-            const uint32_t r = rot_right_small(a, Ll), g = rot_right_small(d, Ll);
-            dnn += popc(a ^ r) + popc(a ^ d);
-            dnnn += popc(a ^ g) + popc(r ^ d);
-            dp += popc(a ^ r ^ d ^ g);
-        }
-    }
+    const uint32_t n = rm_ops_items(Ll);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) rm_ops_item(buf, Ll, i, dnn, dnnn, dp);
     atomicAdd(cnt + 0, dnn);
     atomicAdd(cnt + 1, dnnn);
     atomicAdd(cnt + 2, dp);
@@ -300,36 +263,9 @@ __device__ void rm_ops(const uint32_t *buf, uint32_t Ll, uint32_t *cnt)
 __device__ void rm_block(const uint32_t *src, uint32_t Ll, uint32_t *dst, uint32_t t,
                          uint32_t level, uint32_t k0, uint32_t k1)
 {
-    const uint32_t Lb = Ll >> 1;
-    const uint32_t nb = Lb * Lb;
-    const uint32_t n_out = (nb + 31u) >> 5;
-    for (uint32_t i = threadIdx.x; i < n_out; i += blockDim.x) {
-        const uint32_t coin = tie_coin_word(i, t, level, k0, k1);
-        uint32_t out = 0;
-        if (Lb >= 32u) {
-            const uint32_t wpo = Lb >> 5, wpi = Ll >> 5;
-            const uint32_t by = i / wpo, q = i % wpo;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const uint32_t w0 = src[(2u * by) * wpi + 2u * q + h];
-                const uint32_t w1 = src[(2u * by + 1u) * wpi + 2u * q + h];
-                out |= block16(w0, w1, (coin >> (16 * h)) & 0xFFFFu) << (16 * h);
-            }
-        } else {
-            // rows of Lb < 32 bits: this word holds blocked rows by0 .. by0 + 32/Lb - 1
-            const uint32_t rows = (32u / Lb) < Lb ? (32u / Lb) : Lb;
-            const uint32_t by0 = (32u * i) / Lb;
-            for (uint32_t rr = 0; rr < rows; ++rr) {
-                const uint32_t by = by0 + rr;
-                if (by >= Lb) break;
-                const uint32_t w0 = rm_row_small(src, Ll, 2u * by), w1 = rm_row_small(src, Ll, 2u * by + 1u);
-                const uint32_t off = (by * Lb) & 31u;
-                const uint32_t mask = (1u << Lb) - 1u;
-                out |= (block16(w0, w1, coin >> off) & mask) << off;
-            }
-        }
-        dst[i] = out;
-    }
+    const uint32_t n_out = rm_block_items(Ll);
+    for (uint32_t i = threadIdx.x; i < n_out; i += blockDim.x)
+        dst[i] = rm_block_item(src, Ll, i, tie_coin_word(i, t, level, k0, k1));
 }
 
 // One workgroup per lattice.  Levels 1..nl: level 1 is read from global memory

@@ -1,0 +1,76 @@
+// LDS-resident schedule: bit-packed (CB) lattices held in LDS by a workgroup
+// for a whole chunk of sweeps.  A "slot" is G lattices of one side L and one
+// level count that a workgroup advances together; a persistent grid pulls
+// slots from a queue.  Shared by the host engine and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+#include "hbm_kernels.hpp"
+
+namespace mcrg {
+
+constexpr int kResidentMinL = 8;
+constexpr int kResidentMaxL = 512;
+constexpr uint32_t kMetaWords = 16;
+constexpr uint32_t kMaxLdsBytes = 160u * 1024u - 64u;
+
+// Per-lattice (job) state in device memory.
+struct PackedLattice {
+    uint32_t *spins;      // CB words: colour 0 [W] then colour 1 [W], W = L*L/64
+    int64_t *records;     // max_bins * record_size(n_levels)
+    uint32_t k0, k1;      // Philox key = seed
+    Accept acc;
+    uint32_t t;           // next sweep index
+    uint32_t n_measured;  // measured samples so far
+    uint32_t therm_left;  // unmeasured sweeps still to run
+    uint32_t meas_left;   // measured sweeps still to run
+    int32_t L, n_levels, sweeps_per_bin, max_bins;
+};
+
+// G lattice positions of one (L, n_levels) class: slot_lats[first .. first+G),
+// -1 marks an empty position.
+struct Slot {
+    int32_t L, G, n_levels, first;
+};
+
+__host__ __device__ inline uint32_t cb_words(uint32_t L) { return L * L / 64u; }
+
+__host__ __device__ inline uint32_t level_words(uint32_t L, uint32_t l)
+{
+    const uint32_t Ll = L >> l;
+    const uint32_t w = (Ll * Ll + 31u) / 32u;
+    return w ? w : 1u;
+}
+
+// LDS layout of one slot, offsets in 32-bit words.
+struct SlotLayout {
+    uint32_t W;          // CB words per colour
+    uint32_t lvl_words;  // RM words of levels 1..nl, per lattice
+    uint32_t meta, spins, lvl, cnt, rec, total;
+};
+
+__host__ __device__ inline SlotLayout slot_layout(uint32_t L, uint32_t G, uint32_t nl)
+{
+    SlotLayout s;
+    s.W = cb_words(L);
+    s.lvl_words = 0;
+    for (uint32_t l = 1; l <= nl; ++l) s.lvl_words += level_words(L, l);
+    s.meta = 0;
+    s.spins = s.meta + G * kMetaWords;
+    s.lvl = s.spins + G * 2u * s.W;
+    s.cnt = s.lvl + G * s.lvl_words;
+    s.rec = (s.cnt + G * (nl + 1u) * 4u + 1u) & ~1u;  // int64 aligned
+    s.total = s.rec + G * record_size(nl) * 2u;
+    return s;
+}
+
+hipError_t launch_packed_init(PackedLattice *lats, const int32_t *idx, int n, hipStream_t s);
+hipError_t launch_resident(int threads, const Slot *slots, uint32_t n_slots, const int32_t *slot_lats,
+                           PackedLattice *lats, uint32_t *queue, uint32_t n_chunk, uint32_t lds_bytes,
+                           uint32_t grid, hipStream_t s);
+int resident_max_active_groups(int threads, uint32_t lds_bytes);
+
+}  // namespace mcrg

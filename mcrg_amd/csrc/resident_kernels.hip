@@ -1,0 +1,115 @@
+// LDS-resident schedule kernels (see resident_core.hpp for the phase structure).
+//
+//   k_packed_init   Philox +-1 start of bit-packed (CB) lattices
+//                   (Lattice::initialize_random_spins, src/lattice.cpp:33-41)
+//   k_resident      persistent grid; each workgroup pulls slots (G lattices of
+//                   one L) from an atomic queue and runs one chunk of sweeps and
+//                   measurements per slot with the lattices held in LDS
+//                   (src/ising.cpp:87-149 replaced by Metropolis; src/lattice.cpp:102-120,
+//                   src/mcrg.cpp:72-98 and :314-348 fused)
+#include "resident_core.hpp"
+
+namespace mcrg {
+
+__global__ __launch_bounds__(256) void k_packed_init(PackedLattice *__restrict__ lats,
+                                                     const int32_t *__restrict__ idx)
+{
+    PackedLattice &P = lats[idx[blockIdx.y]];
+    const uint32_t L = (uint32_t)P.L;
+    const uint32_t W = cb_words(L);
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= 2u * W) return;
+    const uint32_t c = j / W, w = j % W;
+    const uint32_t half = L >> 1;
+    uint32_t word = 0;
+    for (uint32_t b = 0; b < 32u; ++b) {
+        const uint32_t mc = 32u * w + b;
+        const uint32_t y = mc / half, k = mc % half;
+        const uint32_t x = 2u * k + ((y + c) & 1u);
+        const uint32_t m = y * L + x;
+        const u32x4 r = philox(m >> 7, 0u, 0u, kTagInit << 28, P.k0, P.k1);
+        word |= ((pick4(r, (m >> 5) & 3u) >> (m & 31u)) & 1u) << b;
+    }
+    P.spins[j] = word;
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_resident(const Slot *__restrict__ slots, uint32_t n_slots,
+                                                 const int32_t *__restrict__ slot_lats,
+                                                 PackedLattice *__restrict__ lats, uint32_t *queue,
+                                                 uint32_t n_chunk)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_idx;
+    for (;;) {
+        if (threadIdx.x == 0) s_idx = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t idx = s_idx;
+        __syncthreads();
+        if (idx >= n_slots) break;
+        run_slot(slots[idx], slot_lats, lats, n_chunk, lds, threadIdx.x, NT);
+    }
+}
+
+__global__ void k_reset_queue(uint32_t *queue) { *queue = 0u; }
+
+hipError_t launch_packed_init(PackedLattice *lats, const int32_t *idx, int n, hipStream_t s)
+{
+    // every lattice in the list is initialised with its own L; the grid covers L = 512
+    const uint32_t max_words = 2u * cb_words((uint32_t)kResidentMaxL);
+    hipLaunchKernelGGL(k_packed_init, dim3((max_words + 255u) / 256u, (uint32_t)n), dim3(256), 0, s,
+                       lats, idx);
+    return hipGetLastError();
+}
+
+template <int NT>
+static hipError_t launch_nt(const Slot *slots, uint32_t n_slots, const int32_t *slot_lats,
+                            PackedLattice *lats, uint32_t *queue, uint32_t n_chunk, uint32_t lds_bytes,
+                            uint32_t grid, hipStream_t s)
+{
+    // above 64 KiB of dynamic LDS the kernel has to be opted in
+    static uint32_t opted_in = 65536u;
+    if (lds_bytes > opted_in) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_resident<NT>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (e != hipSuccess) return e;
+        opted_in = lds_bytes;
+    }
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, queue);
+    hipLaunchKernelGGL(k_resident<NT>, dim3(grid), dim3(NT), lds_bytes, s, slots, n_slots, slot_lats,
+                       lats, queue, n_chunk);
+    return hipGetLastError();
+}
+
+hipError_t launch_resident(int threads, const Slot *slots, uint32_t n_slots, const int32_t *slot_lats,
+                           PackedLattice *lats, uint32_t *queue, uint32_t n_chunk, uint32_t lds_bytes,
+                           uint32_t grid, hipStream_t s)
+{
+    switch (threads) {
+    case 64: return launch_nt<64>(slots, n_slots, slot_lats, lats, queue, n_chunk, lds_bytes, grid, s);
+    case 128: return launch_nt<128>(slots, n_slots, slot_lats, lats, queue, n_chunk, lds_bytes, grid, s);
+    case 256: return launch_nt<256>(slots, n_slots, slot_lats, lats, queue, n_chunk, lds_bytes, grid, s);
+    case 512: return launch_nt<512>(slots, n_slots, slot_lats, lats, queue, n_chunk, lds_bytes, grid, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int resident_max_active_groups(int threads, uint32_t lds_bytes)
+{
+    int per_cu = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (threads) {
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resident<64>, 64, lds_bytes); break;
+    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resident<128>, 128, lds_bytes); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resident<256>, 256, lds_bytes); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resident<512>, 512, lds_bytes); break;
+    default: break;
+    }
+    if (e != hipSuccess || per_cu <= 0) return 0;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return per_cu * cus;
+}
+
+}  // namespace mcrg

@@ -1,0 +1,118 @@
+"""GPU parity of the batched LDS-resident schedule.
+
+Bit-exact against the oracle for mixed L / coupling / level counts / bin sizes,
+for several workgroup sizes and launch lengths; bit-identical to the plain int8
+chain path (a different layout and schedule); and every job's result is the
+same alone and in a mix.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KC = 0.5 * math.log(1.0 + math.sqrt(2.0))
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import mcrg_amd
+    return mcrg_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(amd):
+    c = amd.Context(0)
+    yield c
+    c.close()
+
+
+def _jobs(seed, n, Ls=(8, 16, 32, 64, 128), meas_max=25):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        L = int(rng.choice(Ls))
+        jobs.append(dict(L=L, coupling=float(rng.choice([0.4320459, KC, 0.4496804, 0.4688157, 0.489652, 0.0])),
+                         seed=int(rng.integers(0, 2**63)), n_therm=int(rng.integers(0, 8)),
+                         n_meas=int(rng.integers(0, meas_max)), n_levels=int(rng.integers(0, int(math.log2(L)))),
+                         sweeps_per_bin=int(rng.integers(1, 9))))
+    return jobs
+
+
+def _check_vs_oracle(orc, b, jobs):
+    for j, job in enumerate(jobs):
+        s_ref, r_ref = orc.run_chain(job["L"], job["coupling"], job["seed"], job["n_therm"], job["n_meas"],
+                                     job["n_levels"], job["sweeps_per_bin"])
+        assert np.array_equal(b.spins(j), s_ref), f"spins differ for job {j}: {job}"
+        got = b.records(j)
+        assert np.array_equal(got, r_ref), f"records differ for job {j}: {job} at {np.argwhere(got != r_ref)[:5]}"
+        assert b.state(j) == (job["n_therm"] + job["n_meas"], job["n_meas"])
+
+
+@pytest.mark.parametrize("threads,chunk", [(256, 250), (64, 1), (128, 7), (512, 3)])
+def test_mixed_batch_bit_exact(amd, ctx, orc, threads, chunk):
+    jobs = _jobs(threads + chunk, 40)
+    print("batch", threads, chunk, flush=True)
+    with amd.Batch(ctx, jobs, threads=threads, sweeps_per_launch=chunk) as b:
+        b.run_all()
+        b.sync()
+        _check_vs_oracle(orc, b, jobs)
+
+
+def test_large_lattices_bit_exact(amd, ctx, orc):
+    jobs = [dict(L=256, coupling=KC, seed=11, n_therm=3, n_meas=9, n_levels=6, sweeps_per_bin=4),
+            dict(L=512, coupling=0.4496804, seed=12, n_therm=2, n_meas=6, n_levels=7, sweeps_per_bin=5),
+            dict(L=512, coupling=0.3, seed=13, n_therm=4, n_meas=0, n_levels=2, sweeps_per_bin=1)]
+    with amd.Batch(ctx, jobs, sweeps_per_launch=5) as b:
+        b.run_all()
+        _check_vs_oracle(orc, b, jobs)
+
+
+def test_batch_equals_plain_chain(amd, ctx):
+    """Same seed on two layouts and schedules: int8 HBM chain vs packed LDS batch."""
+    for L, nl in ((16, 3), (64, 4), (128, 5)):
+        job = dict(L=L, coupling=KC, seed=77 + L, n_therm=11, n_meas=40, n_levels=nl, sweeps_per_bin=10)
+        with amd.Batch(ctx, [job], sweeps_per_launch=13) as b:
+            b.run_all()
+            s_b, r_b = b.spins(0), b.records(0)
+        with amd.Chain(ctx, L, KC, job["seed"], n_levels=nl, sweeps_per_bin=10, max_bins=4) as ch:
+            ch.sweep(11)
+            ch.sweep(40, measure=True)
+            assert np.array_equal(ch.spins(), s_b)
+            assert np.array_equal(ch.records(), r_b)
+
+
+def test_alone_equals_in_mix(amd, ctx):
+    probe = dict(L=64, coupling=KC, seed=123456789, n_therm=20, n_meas=60, n_levels=4, sweeps_per_bin=15)
+    with amd.Batch(ctx, [probe], sweeps_per_launch=25) as b:
+        b.run_all()
+        alone = (b.spins(0), b.records(0))
+    mix = _jobs(5, 300, meas_max=80)
+    mix.insert(137, probe)
+    with amd.Batch(ctx, mix, sweeps_per_launch=25) as b:
+        b.run_all()
+        assert np.array_equal(b.spins(137), alone[0])
+        assert np.array_equal(b.records(137), alone[1])
+
+
+def test_incremental_runs(amd, ctx):
+    jobs = _jobs(9, 12)
+    with amd.Batch(ctx, jobs, sweeps_per_launch=4) as a, amd.Batch(ctx, jobs, sweeps_per_launch=4) as b:
+        a.run_all()
+        while b.run(3) > 0:
+            pass
+        for j in range(len(jobs)):
+            assert np.array_equal(a.spins(j), b.spins(j))
+            assert np.array_equal(a.records(j), b.records(j))
+
+
+@pytest.mark.parametrize("bad,msg", [(dict(L=4), "[8, 512]"), (dict(L=1024), "[8, 512]"),
+                                     (dict(n_levels=6), "n_levels"), (dict(coupling=-0.1), "coupling"),
+                                     (dict(sweeps_per_bin=0), "sweeps_per_bin")])
+def test_invalid_jobs(amd, ctx, bad, msg):
+    job = dict(L=32, coupling=KC, seed=1, n_therm=1, n_meas=1, n_levels=2, sweeps_per_bin=1)
+    job.update(bad)
+    with pytest.raises(amd.MCRGError) as e:
+        amd.Batch(ctx, [job])
+    assert e.value.status == amd.ERR_INVALID_ARG and msg in str(e.value)
