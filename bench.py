@@ -1,21 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark of the MCRG hot path on one MI355X.
 
-Workload (plain, HBM-resident): one 2D Ising lattice L = 2048, periodic,
-|K| = 0.4406868, int8 spins, Philox seed 7, checkerboard Metropolis, every
-sweep measured on 6 levels (5 b=2 majority blockings, 2048 -> 64) with the
-NN/NNN/plaquette operators and their cross-correlations in int64.
+Headline workload (batched): the 4096-replica manifest of
+mcrg_amd.workloads.batch_manifest -- L drawn from {8,16,32,64,128}, |K| from
+{0.4320459, 0.4406868, 0.4496804, 0.4688157, 0.489652}, Philox seeds 0..4095,
+bit-packed spins held in LDS, blocking down to L = 4 with int64 NN/NNN/plaquette
+correlations per level, every sweep measured (the measurement phase that is
+50000 of the 55000 sweeps of each replica).  One step = one launch advancing
+every replica by `--sweeps-per-step` measured sweeps.  W untimed warmup steps,
+then exactly K timed steps between device synchronisations.  value = spin
+updates / ns over the timed steps.
 
-One step = one sweep of the lattice plus its measurement.  W untimed warmup
-steps, then exactly K timed steps bracketed by device synchronisation.
-value = spin updates / ns over the timed steps (inputs resident in HBM).
-
-A second, separately timed pass records HIP events around every half-sweep
-launch on the chain's own stream to get the dominant kernel's average
-duration for the roofline entry.  The CPU baseline runs the oracle (a scalar C
-restatement of the same algorithm) on a bounded sample on the host.
+Measured beside it (same JSON line):
+  plain_mixed  the same replicas run one at a time through the plain int8 chain
+               path (one launch per half-sweep), on a sample of the manifest
+  plain_l2048  the plain path on one L = 2048 lattice, 5 blockings
+  roofline     the resident kernel's algorithmic bytes per launch over its
+               HIP-event-timed average duration (events on its own stream)
+  cpu_baseline the oracle (scalar C) on the host's cores over a sample of the
+               manifest (threads over independent replicas)
 """
 import argparse
+import concurrent.futures as cf
 import json
 import math
 import os
@@ -33,73 +39,137 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(L, J, seed, nl, n_sweeps):
+def bytes_per_update(measured):
+    # bit-packed: 3 * (1/8) B per site update, plus (5/3) * (1/8) B per site per
+    # measured sweep (level-0 read + the geometric series of blocked levels)
+    return 3.0 / 8.0 + (5.0 / 24.0 if measured else 0.0)
+
+
+def run_batched(mcrg_amd, ctx, jobs, steps, warmup, sps, threads):
+    n_sweeps = (warmup + steps) * sps
+    bjobs = [dict(j, n_therm=0, n_meas=n_sweeps, sweeps_per_bin=max(1, n_sweeps)) for j in jobs]
+    sites = sum(j["L"] * j["L"] for j in bjobs)
+    with mcrg_amd.Batch(ctx, bjobs, threads=threads, sweeps_per_launch=sps) as b:
+        b.run(warmup * sps)
+        b.sync()
+        t0 = time.perf_counter()
+        b.run(steps * sps)
+        b.sync()
+        dt = time.perf_counter() - t0
+    return sites * sps * steps / (dt * 1e9), dt, sites
+
+
+def run_batched_timed(mcrg_amd, ctx, jobs, sps, threads, launches):
+    """Kernel duration of the resident kernel from HIP events on its stream."""
+    n_sweeps = launches * sps
+    bjobs = [dict(j, n_therm=0, n_meas=n_sweeps, sweeps_per_bin=max(1, n_sweeps)) for j in jobs]
+    with mcrg_amd.Batch(ctx, bjobs, threads=threads, sweeps_per_launch=sps) as b:
+        b.run(sps)  # first launch builds the slot table
+        b.sync()
+        n0, ms0 = ctx.kernel_time(3)
+        ctx.set_timing(True)
+        b.run((launches - 1) * sps)
+        b.sync()
+        n1, ms1 = ctx.kernel_time(3)
+        ctx.set_timing(False)
+    return (ms1 - ms0) / max(1, n1 - n0), n1 - n0
+
+
+def run_plain_mixed(mcrg_amd, ctx, jobs, sweeps):
+    """Plain path, one replica at a time: updates/ns over the whole sample."""
+    updates = 0
+    t0 = time.perf_counter()
+    for j in jobs:
+        with mcrg_amd.Chain(ctx, j["L"], j["coupling"], j["seed"], n_levels=j["n_levels"],
+                            sweeps_per_bin=sweeps, max_bins=1) as ch:
+            ch.sweep(sweeps, measure=True)
+            ch.sync()
+        updates += j["L"] * j["L"] * sweeps
+    dt = time.perf_counter() - t0
+    return updates / (dt * 1e9), dt
+
+
+def run_plain_2048(mcrg_amd, ctx, steps):
+    L, J, seed, nl = 2048, 0.4406868, 7, 5
+    with mcrg_amd.Chain(ctx, L, J, seed, n_levels=nl, sweeps_per_bin=steps + 20, max_bins=1) as ch:
+        ch.sweep(20, measure=True)
+        ch.sync()
+        t0 = time.perf_counter()
+        ch.sweep(steps, measure=True)
+        ch.sync()
+        dt = time.perf_counter() - t0
+    return L * L * steps / (dt * 1e9)
+
+
+def cpu_baseline(jobs, sweeps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
     orc.lib()
+
+    def one(j):
+        orc.run_chain(j["L"], j["coupling"], j["seed"], 0, sweeps, j["n_levels"], sweeps)
+        return j["L"] * j["L"] * sweeps
+
     t0 = time.perf_counter()
-    orc.run_chain(L, J, seed, 0, n_sweeps, nl, n_sweeps)
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:  # ctypes releases the GIL
+        updates = sum(ex.map(one, jobs))
     dt = time.perf_counter() - t0
-    return {"value": L * L * n_sweeps / (dt * 1e9), "unit": "spin-updates/ns", "cores": 1,
-            "kind": "port",
-            "sample": f"oracle (scalar C, -O2) L={L} {n_sweeps} measured sweeps, {nl} levels: "
-                      f"{dt:.2f} s on 1 host core"}
+    return {"value": updates / (dt * 1e9), "unit": "spin-updates/ns", "cores": threads, "kind": "port",
+            "sample": f"oracle (scalar C, -O2) on {len(jobs)} replicas of the batching manifest x {sweeps} "
+                      f"measured sweeps each, {threads} threads over independent replicas: {dt:.1f} s"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--timing-steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--sweeps-per-step", type=int, default=100)
+    ap.add_argument("--threads", type=int, default=256)
+    ap.add_argument("--replicas", type=int, default=4096)
+    ap.add_argument("--plain-sample", type=int, default=50)
     ap.add_argument("--cpu-sweeps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the per-GPU work is "
-              "independent chains, so there is nothing to exchange across GPUs. Run with --gpus 1.",
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the replicas are "
+              "independent chains and this build runs on a single device. Run with --gpus 1.",
               file=sys.stderr)
         return 2
 
     import mcrg_amd
+    from mcrg_amd import workloads
 
-    L, J, seed, nl = 2048, 0.4406868, 7, 5
-    spb = 1000
-    total = args.warmup + args.steps + args.timing_steps
+    jobs = workloads.batch_manifest(args.replicas)
     ctx = mcrg_amd.Context(0)
-    ch = mcrg_amd.Chain(ctx, L, J, seed, n_levels=nl, sweeps_per_bin=spb,
-                        max_bins=max(1, math.ceil(total / spb)))
-    log(f"bench: L={L} warmup={args.warmup} steps={args.steps}")
-    ch.sweep(args.warmup, measure=True)
-    ch.sync()
-    t0 = time.perf_counter()
-    ch.sweep(args.steps, measure=True)
-    ch.sync()
-    dt = time.perf_counter() - t0
-    updates = L * L * args.steps
-    value = updates / (dt * 1e9)
-    log(f"bench: timed {args.steps} steps in {dt:.3f} s -> {value:.1f} updates/ns")
+    dev = ctx.device_info()
+    log(f"bench: {dev['name']} {dev['compute_units']} CUs; {len(jobs)} replicas, "
+        f"{args.warmup}+{args.steps} steps x {args.sweeps_per_step} sweeps")
+    value, dt, sites = run_batched(mcrg_amd, ctx, jobs, args.steps, args.warmup, args.sweeps_per_step,
+                                   args.threads)
+    log(f"bench: batched {value:.1f} updates/ns ({dt:.3f} s timed)")
 
-    # dominant kernel: the half-sweep, timed with events on the chain's stream
-    ctx.set_timing(True)
-    ch.sweep(args.timing_steps, measure=True)
-    ch.sync()
-    n_launch, ms = ctx.kernel_time(0)
-    n_m0, ms_m0 = ctx.kernel_time(1)
-    n_lv, ms_lv = ctx.kernel_time(2)
-    ctx.set_timing(False)
-    avg_us = 1e3 * ms / max(1, n_launch)
-    bytes_per_launch = 1.5 * L * L  # read all L^2 sites, write L^2/2 (int8)
-    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
-    log(f"bench: half-sweep avg {avg_us:.2f} us over {n_launch} launches -> {achieved:.1f} GB/s")
+    avg_ms, n_launch = run_batched_timed(mcrg_amd, ctx, jobs, args.sweeps_per_step, args.threads, 11)
+    bpl = bytes_per_update(True) * sites * args.sweeps_per_step
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
+    log(f"bench: resident kernel {avg_ms:.3f} ms/launch over {n_launch} launches -> {achieved:.1f} GB/s")
 
-    ch.close()
+    plain_mixed = plain_2048 = None
+    if not args.no_plain:
+        sample = jobs[:args.plain_sample]
+        plain_mixed, pdt = run_plain_mixed(mcrg_amd, ctx, sample, 20)
+        log(f"bench: plain path on {len(sample)} replicas x 20 sweeps: {plain_mixed:.3f} updates/ns")
+        plain_2048 = run_plain_2048(mcrg_amd, ctx, 500)
+        log(f"bench: plain L=2048: {plain_2048:.1f} updates/ns")
     ctx.close()
 
     cpu = None
     if not args.no_cpu:
-        log(f"bench: cpu baseline {args.cpu_sweeps} sweeps")
-        cpu = cpu_baseline(L, J, seed, nl, args.cpu_sweeps)
+        threads = min(16, os.cpu_count() or 1)
+        sample = jobs[:1024]
+        log(f"bench: cpu baseline on {len(sample)} replicas x {args.cpu_sweeps} sweeps, {threads} threads")
+        cpu = cpu_baseline(sample, args.cpu_sweeps, threads)
 
     out = {
         "metric": METRIC,
@@ -112,17 +182,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8",
+        "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": "plain HBM int8 L=2048, |K|=0.4406868, seed 7, 5 blockings (2048->64), "
-                               "measure every sweep", "L": L, "coupling": J, "levels": nl,
-                   "layout": "int8", "schedule": "hbm"},
+        "config": {"workload": f"batched: {len(jobs)} replicas, L in {{8..128}}, 5 couplings, seeds 0..{len(jobs)-1}, "
+                               f"bit-packed LDS-resident, blocking to L=4, every sweep measured",
+                   "replicas": len(jobs), "sweeps_per_step": args.sweeps_per_step, "threads": args.threads,
+                   "sites": sites, "layout": "packed", "schedule": "resident", "device": dev["name"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_int8_half_sweep", "avg_us": avg_us,
-                     "bytes_per_launch": bytes_per_launch},
-        "kernels_us": {"half_sweep": avg_us, "measure0": 1e3 * ms_m0 / max(1, n_m0),
-                       "levels_accumulate": 1e3 * ms_lv / max(1, n_lv)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_resident<256>",
+                     "avg_ms": avg_ms, "bytes_per_launch": bpl},
+        "plain_mixed": plain_mixed,
+        "plain_l2048": plain_2048,
+        "batched_over_plain_mixed": (value / plain_mixed) if plain_mixed else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -3,8 +3,10 @@
 // Stream layout (identical for every schedule and storage layout, so that any
 // schedule reproduces the same chain bit for bit):
 //   key      (lo32(seed), hi32(seed))
-//   sweep    Philox(x0 = word, x1 = t, x2 = slice group j, x3 = 1<<28 | colour<<24);
-//            out[q] is the 32-bit random "slice" 4j+q of the word's 32 sites
+//   sweep    Philox(x0 = word, x1 = t, x2 = 0, x3 = 1<<28 | colour<<24): out[q] is
+//            bit 31-q of every site's 32-bit uniform; sites whose 4 top bits tie
+//            with the threshold's take the top 28 bits of word rho & 3 of
+//            Philox(word, t, 1 + (rho >> 2), 1<<28 | colour<<24), rho = their rank
 //   tie coin Philox(x0 = bm>>7, x1 = t, 0, 2<<28 | level<<16); bit bm&31 of out[(bm>>5)&3]
 //   init     Philox(x0 = m>>7, 0, 0, 3<<28);                  bit m&31 of out[(m>>5)&3]
 // Site bits: b = 1 for spin -1 (the sign bit of the int8 value), b = 0 for +1.
@@ -61,11 +63,12 @@ struct Accept {
 // spin bits of the sites and of their up/down/left/right neighbours; `valid`
 // masks the sites that exist.  Returns the bits to flip.
 //
-// e <= 0 flips outright.  The others compare U_i < thr lazily, most significant
-// bit first: slice k supplies bit 31-k of every U_i at once, and a site is
-// decided at its first slice whose bit differs from the threshold's.  The result
-// equals the full 32-bit comparison; later slices are never needed once every
-// site of the word is decided.
+// e <= 0 flips outright.  The other sites compare a 32-bit uniform U_i with the
+// threshold of their class.  The first Philox call gives every site the 4 top
+// bits of its U_i (slice q = out[q] supplies bit 31-q of all 32 sites), which
+// decide the comparison unless they equal the threshold's 4 top bits
+// (probability 1/16).  The sites still open, ranked by bit index, take their 28
+// low bits from refinement calls, four sites per call.
 __device__ __forceinline__ uint32_t metropolis_word(uint32_t s, uint32_t nu, uint32_t nd,
                                                     uint32_t nl, uint32_t nr, uint32_t valid,
                                                     const Accept &acc, uint32_t k0, uint32_t k1,
@@ -80,20 +83,28 @@ __device__ __forceinline__ uint32_t metropolis_word(uint32_t s, uint32_t nu, uin
     flip |= (acc.always0 ? c1 : 0u) | (acc.always1 ? c2 : 0u);
     const uint32_t m1 = acc.always0 ? 0u : c1;
     const uint32_t m2 = acc.always1 ? 0u : c2;
-    uint32_t undecided = (m1 | m2) & valid;
+    uint32_t open = (m1 | m2) & valid;
     uint32_t accept = 0;
-    const uint32_t tag = (kTagSweep << 28) | (colour << 24);
-    for (uint32_t j = 0; j < 8u && undecided != 0u; ++j) {
-        const u32x4 r = philox(word, t, j, tag, k0, k1);
+    if (open != 0u) {
+        const uint32_t tag = (kTagSweep << 28) | (colour << 24);
+        const u32x4 r = philox(word, t, 0u, tag, k0, k1);
         const uint32_t rq[4] = {r.v0, r.v1, r.v2, r.v3};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const uint32_t k = 4u * j + (uint32_t)q;
-            const uint32_t b0 = 0u - ((acc.thr0 >> (31u - k)) & 1u);
-            const uint32_t b1 = 0u - ((acc.thr1 >> (31u - k)) & 1u);
+            const uint32_t b0 = 0u - ((acc.thr0 >> (31 - q)) & 1u);
+            const uint32_t b1 = 0u - ((acc.thr1 >> (31 - q)) & 1u);
             const uint32_t tb = (m1 & b0) | (m2 & b1);
-            accept |= undecided & ~rq[q] & tb;
-            undecided &= ~(rq[q] ^ tb);
+            accept |= open & ~rq[q] & tb;
+            open &= ~(rq[q] ^ tb);
+        }
+        u32x4 ref = {0u, 0u, 0u, 0u};
+        for (uint32_t rank = 0; open != 0u; ++rank) {
+            const uint32_t i = (uint32_t)__builtin_ctz(open);
+            open &= open - 1u;
+            if ((rank & 3u) == 0u) ref = philox(word, t, 1u + (rank >> 2), tag, k0, k1);
+            const uint32_t low = pick4(ref, rank & 3u) >> 4;
+            const uint32_t thr = ((m1 >> i) & 1u) ? acc.thr0 : acc.thr1;
+            accept |= (low < (thr & 0x0FFFFFFFu) ? 1u : 0u) << i;
         }
     }
     return (flip | accept) & valid;

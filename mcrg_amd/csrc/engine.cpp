@@ -123,14 +123,17 @@ size_t ResidentEngine::job_bytes(const JobSpec &j)
 
 uint32_t ResidentEngine::lds_bytes(int L, int G, int nl) const
 {
-    return slot_layout((uint32_t)L, (uint32_t)G, (uint32_t)nl).total * 4u;
+    const uint32_t T = (uint32_t)opt_.threads / (uint32_t)G;
+    return slot_layout((uint32_t)L, (uint32_t)G, (uint32_t)nl, T).total * 4u;
 }
 
 int ResidentEngine::group_size(int L, int nl) const
 {
     const int W = (int)cb_words((uint32_t)L);
     int G = W >= opt_.threads ? 1 : opt_.threads / W;
-    G = std::min(G, opt_.max_group);
+    int cap = 1;
+    while (cap * 2 <= opt_.max_group) cap *= 2;  // positions are a power of two
+    G = std::min(G, cap);
     // keep a slot's LDS modest so that several workgroups share a CU
     const uint32_t soft = 64u * 1024u;
     while (G > 1 && lds_bytes(L, G, nl) > soft) G /= 2;
@@ -275,16 +278,23 @@ void ResidentEngine::rebuild_slots(uint32_t chunk)
         const std::vector<int> &ids = kv.second;
         for (size_t a = 0; a < ids.size(); a += (size_t)G) {
             const int g = (int)std::min<size_t>((size_t)G, ids.size() - a);
-            Slot s{L, g, nl, (int32_t)slot_lats_.size()};
+            // positions: a power of two (so that teams tile the workgroup), empty ones are -1
+            int gp = 1;
+            while (gp < g) gp *= 2;
+            Slot s{L, gp, nl, (int32_t)slot_lats_.size()};
             uint64_t work = 0;
-            for (int k = 0; k < g; ++k) {
+            for (int k = 0; k < gp; ++k) {
+                if (k >= g) {
+                    slot_lats_.push_back(-1);
+                    continue;
+                }
                 const int id = ids[a + (size_t)k];
                 slot_lats_.push_back(id);
                 const Job &j = jobs_[(size_t)id];
                 work += (uint64_t)L * (uint64_t)L * std::min<uint64_t>(chunk, j.therm_left + j.meas_left);
             }
             pend.push_back({s, work});
-            max_lds_ = std::max(max_lds_, lds_bytes(L, g, nl));
+            max_lds_ = std::max(max_lds_, lds_bytes(L, gp, nl));
         }
     }
     // longest slots first: the persistent grid pulls them in this order

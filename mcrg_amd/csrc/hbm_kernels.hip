@@ -64,9 +64,9 @@ __global__ __launch_bounds__(256) void k_int8_half_sweep(const HbmLattice *__res
     const Accept acc = lat.acc;
     int8_t *spins = lat.spins;
     if (L >= 64) {
-        const uint32_t wpr = L >> 6;
+        const uint32_t wpr = L >> 6, sh = (uint32_t)__builtin_ctz(wpr);
         if (w >= L * wpr) return;
-        const uint32_t y = w / wpr, q = w % wpr;
+        const uint32_t y = w >> sh, q = w & (wpr - 1u);
         const uint32_t p = (y + colour) & 1u;
         int8_t *row = spins + (size_t)y * L + 64u * q;
         const int8_t *up = spins + (size_t)((y - 1u) & (L - 1u)) * L + 64u * q;
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_int8_measure0(const HbmLattice *__restr
         const uint32_t wpr = L >> 6;  // 64-site chunks per row == level-1 words per row
         const uint32_t n_items = (L >> 1) * wpr;
         if (i < n_items) {
-            const uint32_t by = i / wpr, q = i % wpr;
+            const uint32_t by = i >> __builtin_ctz(wpr), q = i & (wpr - 1u);
             uint32_t lo[3], hi[3], nx[3];
 #pragma unroll
             for (int r = 0; r < 3; ++r) {

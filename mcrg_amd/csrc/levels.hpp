@@ -17,6 +17,9 @@
 
 namespace mcrg {
 
+// log2 of a power of two
+__device__ __forceinline__ uint32_t ilog2u(uint32_t v) { return (uint32_t)__builtin_ctz(v); }
+
 // ------------------------------------------------------------------ RM levels
 __device__ __forceinline__ uint32_t rm_row_small(const uint32_t *buf, uint32_t Ll, uint32_t y)
 {
@@ -39,8 +42,8 @@ __device__ __forceinline__ void rm_ops_item(const uint32_t *buf, uint32_t Ll, ui
 {
     uint32_t a, r, d, g;
     if (Ll >= 32u) {
-        const uint32_t wpr = Ll >> 5;
-        const uint32_t y = i / wpr, q = i % wpr;
+        const uint32_t wpr = Ll >> 5, sh = ilog2u(wpr);
+        const uint32_t y = i >> sh, q = i & (wpr - 1u);
         const uint32_t yn = (y + 1u) & (Ll - 1u), qn = (q + 1u) & (wpr - 1u);
         a = buf[y * wpr + q];
         const uint32_t an = buf[y * wpr + qn];
@@ -75,7 +78,7 @@ __device__ __forceinline__ uint32_t rm_block_item(const uint32_t *src, uint32_t 
     uint32_t out = 0;
     if (Lb >= 32u) {
         const uint32_t wpo = Lb >> 5, wpi = Ll >> 5;
-        const uint32_t by = i / wpo, q = i % wpo;
+        const uint32_t by = i >> ilog2u(wpo), q = i & (wpo - 1u);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const uint32_t w0 = src[(2u * by) * wpi + 2u * q + (uint32_t)h];
@@ -84,8 +87,9 @@ __device__ __forceinline__ uint32_t rm_block_item(const uint32_t *src, uint32_t 
         }
     } else {
         // rows of Lb < 32 bits: this word holds blocked rows by0 .. by0 + 32/Lb - 1
-        const uint32_t rows = (32u / Lb) < Lb ? (32u / Lb) : Lb;
-        const uint32_t by0 = (32u * i) / Lb;
+        const uint32_t lb = ilog2u(Lb);
+        const uint32_t rows = (32u >> lb) < Lb ? (32u >> lb) : Lb;
+        const uint32_t by0 = (32u * i) >> lb;
         const uint32_t mask = (1u << Lb) - 1u;
         for (uint32_t rr = 0; rr < rows; ++rr) {
             const uint32_t by = by0 + rr;
@@ -153,13 +157,13 @@ __device__ __forceinline__ Nbr4 cb_neighbours(const uint32_t *oth, uint32_t L, u
 {
     Nbr4 n;
     if (L >= 64u) {
-        const uint32_t wpr = L >> 6;
-        const uint32_t y = w / wpr, q = w % wpr;
+        const uint32_t wpr = L >> 6, sh = ilog2u(wpr);
+        const uint32_t y = w >> sh, q = w & (wpr - 1u);
         const uint32_t p = (y + c) & 1u;
-        const uint32_t row = y * wpr;
+        const uint32_t row = y << sh;
         const uint32_t o = oth[row + q];
-        n.nu = oth[((y - 1u) & (L - 1u)) * wpr + q];
-        n.nd = oth[((y + 1u) & (L - 1u)) * wpr + q];
+        n.nu = oth[(((y - 1u) & (L - 1u)) << sh) + q];
+        n.nd = oth[(((y + 1u) & (L - 1u)) << sh) + q];
         if (p == 0u) {
             n.nr = o;
             n.nl = (o << 1) | (oth[row + ((q - 1u) & (wpr - 1u))] >> 31);
@@ -192,10 +196,10 @@ __device__ __forceinline__ void cb_ops_item(const uint32_t *own, const uint32_t 
     const Nbr4 n = cb_neighbours(oth, L, W, w, c);
     uint32_t dr, dl;  // same colour, row y+1: (x+1, y+1) and (x-1, y+1)
     if (L >= 64u) {
-        const uint32_t wpr = L >> 6;
-        const uint32_t y = w / wpr, q = w % wpr;
+        const uint32_t wpr = L >> 6, sh = ilog2u(wpr);
+        const uint32_t y = w >> sh, q = w & (wpr - 1u);
         const uint32_t p = (y + c) & 1u;
-        const uint32_t row = ((y + 1u) & (L - 1u)) * wpr;
+        const uint32_t row = ((y + 1u) & (L - 1u)) << sh;
         const uint32_t nx = own[row + q];
         if (p == 0u) {
             dr = nx;
@@ -229,9 +233,9 @@ __device__ __forceinline__ uint32_t cb_block_item(const uint32_t *c0, const uint
                                                   uint32_t W, uint32_t i, uint32_t coin)
 {
     if (L >= 64u) {
-        const uint32_t wpr = L >> 6;
-        const uint32_t by = i / wpr, q = i % wpr;
-        const uint32_t r0 = (2u * by) * wpr + q, r1 = (2u * by + 1u) * wpr + q;
+        const uint32_t wpr = L >> 6, sh = ilog2u(wpr);
+        const uint32_t by = i >> sh, q = i & (wpr - 1u);
+        const uint32_t r0 = ((2u * by) << sh) + q, r1 = ((2u * by + 1u) << sh) + q;
         const uint32_t b1 = c0[r0], b2 = c1[r0], b3 = c0[r1], b4 = c1[r1];
         const uint32_t maj3 = (b1 & b2 & (b3 | b4)) | (b3 & b4 & (b1 | b2));
         const uint32_t tie = ~(b1 ^ b2 ^ b3 ^ b4) & ~(b1 & b2 & b3 & b4) & (b1 | b2 | b3 | b4);

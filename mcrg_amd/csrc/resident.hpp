@@ -45,24 +45,35 @@ __host__ __device__ inline uint32_t level_words(uint32_t L, uint32_t l)
     return w ? w : 1u;
 }
 
-// LDS layout of one slot, offsets in 32-bit words.
+// LDS layout of one slot, offsets in 32-bit words.  T = threads per lattice.
 struct SlotLayout {
-    uint32_t W;          // CB words per colour
-    uint32_t lvl_words;  // RM words of levels 1..nl, per lattice
-    uint32_t meta, spins, lvl, cnt, rec, total;
+    uint32_t W;           // CB words per colour
+    uint32_t lvl_words;   // RM words of levels 1..nl, per lattice (also the coin words)
+    uint32_t team_waves;  // waves per lattice team (count partials per level)
+    uint32_t n_sv;        // int64 operator values per lattice: 3 (nl+1) + M, |M|, 1
+    uint32_t coin_calls;  // Philox calls for all tie-coin words (4 words per call)
+    uint32_t meta, spins, lvl, coin, cnt, sv, rec, total;
 };
 
-__host__ __device__ inline SlotLayout slot_layout(uint32_t L, uint32_t G, uint32_t nl)
+__host__ __device__ inline SlotLayout slot_layout(uint32_t L, uint32_t G, uint32_t nl, uint32_t T)
 {
     SlotLayout s;
     s.W = cb_words(L);
     s.lvl_words = 0;
-    for (uint32_t l = 1; l <= nl; ++l) s.lvl_words += level_words(L, l);
+    s.coin_calls = 0;
+    for (uint32_t l = 1; l <= nl; ++l) {
+        s.lvl_words += level_words(L, l);
+        s.coin_calls += (level_words(L, l) + 3u) / 4u;
+    }
+    s.team_waves = T > 64u ? T / 64u : 1u;
     s.meta = 0;
     s.spins = s.meta + G * kMetaWords;
     s.lvl = s.spins + G * 2u * s.W;
-    s.cnt = s.lvl + G * s.lvl_words;
-    s.rec = (s.cnt + G * (nl + 1u) * 4u + 1u) & ~1u;  // int64 aligned
+    s.coin = s.lvl + G * s.lvl_words;
+    s.cnt = s.coin + G * s.lvl_words;
+    s.n_sv = 3u * (nl + 1u) + 3u;
+    s.sv = (s.cnt + G * s.team_waves * (nl + 1u) * 4u + 1u) & ~1u;  // int64 aligned
+    s.rec = s.sv + G * s.n_sv * 2u;
     s.total = s.rec + G * record_size(nl) * 2u;
     return s;
 }

@@ -1,37 +1,29 @@
 // Body of the LDS-resident schedule: one workgroup advances the G lattices of
 // one slot by up to n_chunk sweeps, measuring the sweeps that fall in each
-// lattice's measurement phase.  Phases are loops over work items separated by
-// workgroup barriers; every item is owned by exactly one thread, so no phase
-// depends on which thread runs which item.
+// lattice's measurement phase.
 //
-//   sweep      colour 0 then colour 1: one item per CB word, Metropolis in place
-//   measure    level-0 operators + blocking to level 1 (CB -> RM), then levels
-//              1..nl (RM), then one item per record entry adds the sample's
-//              sums and products to the LDS partial record
-//   flush      partial records are added to the per-bin records in HBM when a
-//              lattice's sample index enters a new bin and at the chunk end
+// Each lattice is owned by a team of T = nt / G consecutive threads (lane u of
+// lattice g is thread g*T + u).  A team of at most 64 lanes sits inside one
+// wave and synchronises at wave level only; larger teams (L >= 128) use the
+// workgroup barrier.  Per measured sweep:
+//   coins    every tie-coin word of every blocking level, one Philox call per
+//            4 consecutive coin words (the coins do not depend on the spins)
+//   level 0  operator counts of the CB lattice, blocking into the RM level 1
+//   levels   operator counts and blocking of RM levels 1..nl
+//   record   team-wide sums of the per-lane counts, then one lane per record
+//            entry adds the sample's sums and products to the LDS partial
+//            record, which is flushed into the per-bin record in HBM when the
+//            lattice's sample index enters a new bin and at the chunk end.
+//
+// Loops are written as `for g` / `for u` with strides (gs, us): on the GPU they
+// run once per thread (g = tid / T, u = tid % T); the CPU emulation used while
+// developing runs them with stride 1 and no synchronisation.
 #pragma once
 
 #include "levels.hpp"
 #include "resident.hpp"
 
-#ifndef MCRG_SYNC
-#define MCRG_SYNC() __syncthreads()
-#define MCRG_SYNC_OR(x) __syncthreads_or(x)
-#endif
-
 namespace mcrg {
-
-#ifndef MCRG_HOST_EMULATION
-// Sum over aligned groups of `width` lanes (a power of two <= 64); every lane of
-// the wave must execute it.
-__device__ __forceinline__ uint32_t group_sum(uint32_t v, uint32_t width)
-{
-    for (uint32_t off = 1; off < width; off <<= 1) v += __shfl_xor(v, (int)off, 64);
-    return v;
-}
-__device__ __forceinline__ bool group_leader(uint32_t tid, uint32_t width) { return (tid & (width - 1u)) == 0u; }
-#endif
 
 // Meta words per lattice position (LDS).
 enum : uint32_t {
@@ -49,36 +41,46 @@ enum : uint32_t {
     kMetaAlw1 = 11,
 };
 
-__device__ __forceinline__ int64_t level_S(const uint32_t *cg, uint32_t L, uint32_t l, uint32_t a)
+struct Team {
+    uint32_t g0, gs, u0, us;  // loop starts and strides over lattices and lanes
+    uint32_t T;               // lanes per lattice
+    uint32_t lane;            // thread index within the workgroup
+    bool block;               // team spans more than one wave
+};
+
+#ifndef MCRG_HOST_EMULATION
+__device__ __forceinline__ void team_sync(const Team &tm)
 {
-    const int64_t n = (int64_t)(L >> l) * (int64_t)(L >> l);
-    const int64_t d = (int64_t)cg[l * 4u + a];
-    return a < 2u ? 2 * n - 2 * d : n - 2 * d;
+    if (tm.block) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 }
 
-// Increment of record entry k for one sample (layout of record_size()).
-__device__ __forceinline__ int64_t record_term(const uint32_t *cg, uint32_t L, uint32_t nl, uint32_t k)
+__device__ __forceinline__ bool team_any(const Team &tm, bool p)
 {
-    const uint32_t n_s = 3u * (nl + 1u), n_self = 9u * (nl + 1u), n_cross = 9u * nl;
-    if (k < n_s) return level_S(cg, L, k / 3u, k % 3u);
-    k -= n_s;
-    if (k < n_self) {
-        const uint32_t l = k / 9u, r = k % 9u;
-        return level_S(cg, L, l, r % 3u) * level_S(cg, L, l, r / 3u);
-    }
-    k -= n_self;
-    if (k < n_cross) {
-        const uint32_t l = k / 9u + 1u, r = k % 9u;
-        return level_S(cg, L, l, r % 3u) * level_S(cg, L, l - 1u, r / 3u);
-    }
-    k -= n_cross;
-    const int64_t n0 = (int64_t)L * (int64_t)L;
-    const int64_t M = n0 - 2 * (int64_t)cg[3];
-    if (k == 0u) return M;
-    if (k == 1u) return M < 0 ? -M : M;
-    if (k == 2u) return M * M;
-    return 1;
+    if (tm.block) return __syncthreads_or(p ? 1 : 0) != 0;
+    return __ballot(p ? 1 : 0) != 0ull;
 }
+
+__device__ __forceinline__ void block_sync() { __syncthreads(); }
+
+// Sum over the aligned group of `width` lanes (1..64, a power of two) that
+// contains this lane; for width 64 the total is valid in lanes 0..31.
+__device__ __forceinline__ uint32_t group_sum(uint32_t v, uint32_t width)
+{
+    if (width >= 2u) v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+    if (width >= 4u) v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+    if (width >= 8u) v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false); // row_half_mirror
+    if (width >= 16u) v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false); // row_mirror
+    if (width >= 32u) v += (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);             // xor 16
+    if (width >= 64u) v += (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+    return v;
+}
+#endif
 
 __device__ __forceinline__ bool in_run(const uint32_t *m, uint32_t it)
 {
@@ -89,211 +91,445 @@ __device__ __forceinline__ bool in_meas(const uint32_t *m, uint32_t it)
     return it >= m[kMetaTherm] && it < m[kMetaTherm] + m[kMetaMeas];
 }
 
-__device__ __forceinline__ uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
-
-// Adds `v` of item i (items of one lattice are contiguous, `per` per lattice)
-// into counter slot `dst[g]`.  Groups of lanes that belong to the same lattice
-// reduce first so that one lane per group touches LDS.
-__device__ __forceinline__ void add_count(uint32_t *cnt, uint32_t stride, uint32_t i, uint32_t per,
-                                          uint32_t n_items, uint32_t v, uint32_t tid)
+// Operator counts of lattice g at level l: sum over the team's wave partials.
+__device__ __forceinline__ uint32_t count_of(const uint32_t *cntg, uint32_t nw, uint32_t nl, uint32_t l,
+                                             uint32_t a)
 {
-    const uint32_t width = per < 64u ? per : 64u;
-    v = group_sum(v, width);
-    if (group_leader(tid, width) && i < n_items) atomicAdd(cnt + (i / per) * stride, v);
+    uint32_t s = 0;
+    for (uint32_t w = 0; w < nw; ++w) s += cntg[(w * (nl + 1u) + l) * 4u + a];
+    return s;
 }
 
+__device__ __forceinline__ int64_t level_S(const uint32_t *cntg, uint32_t nw, uint32_t nl, uint32_t L,
+                                           uint32_t l, uint32_t a)
+{
+    const int64_t n = (int64_t)(L >> l) * (int64_t)(L >> l);
+    const int64_t d = (int64_t)count_of(cntg, nw, nl, l, a);
+    return a < 2u ? 2 * n - 2 * d : n - 2 * d;
+}
+
+// Record entry k of one sample is sv[x] * sv[y] over the lattice's operator
+// values sv = (S[0][0..2], ..., S[nl][0..2], M, |M|, 1) (layout of record_size()).
+__device__ __forceinline__ void record_pair(uint32_t nl, uint32_t k, uint32_t &x, uint32_t &y)
+{
+    const uint32_t n_s = 3u * (nl + 1u), n_self = 9u * (nl + 1u), n_cross = 9u * nl;
+    const uint32_t iM = n_s, iAbs = n_s + 1u, iOne = n_s + 2u;
+    if (k < n_s) {
+        x = k;
+        y = iOne;
+    } else if (k < n_s + n_self) {
+        const uint32_t kk = k - n_s, l = kk / 9u, r = kk % 9u;
+        x = 3u * l + r % 3u;
+        y = 3u * l + r / 3u;
+    } else if (k < n_s + n_self + n_cross) {
+        const uint32_t kk = k - n_s - n_self, l = kk / 9u + 1u, r = kk % 9u;
+        x = 3u * l + r % 3u;
+        y = 3u * (l - 1u) + r / 3u;
+    } else {
+        const uint32_t m = k - n_s - n_self - n_cross;
+        x = m == 0u ? iM : m == 1u ? iAbs : m == 2u ? iM : iOne;
+        y = m == 2u ? iM : iOne;
+    }
+}
+
+// Operator value i of sv (see record_pair) from the counts.
+__device__ __forceinline__ int64_t sv_value(const uint32_t *cntg, uint32_t nw, uint32_t nl, uint32_t L, uint32_t i)
+{
+    const uint32_t n_s = 3u * (nl + 1u);
+    if (i < n_s) return level_S(cntg, nw, nl, L, i / 3u, i % 3u);
+    if (i == n_s + 2u) return 1;
+    const int64_t n0 = (int64_t)L * (int64_t)L;
+    const int64_t M = n0 - 2 * (int64_t)count_of(cntg, nw, nl, 0u, 3u);
+    return i == n_s ? M : (M < 0 ? -M : M);
+}
+
+// Team-wide reduction of one level's counts into the lattice's wave partial.
+__device__ __forceinline__ void store_counts(uint32_t *cntg, const Team &tm, uint32_t nl, uint32_t l,
+                                             uint32_t u, uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3)
+{
+    const uint32_t width = tm.T < 64u ? tm.T : 64u;
+#ifndef MCRG_HOST_EMULATION
+    d0 = group_sum(d0, width);
+    d1 = group_sum(d1, width);
+    d2 = group_sum(d2, width);
+    d3 = group_sum(d3, width);
+    if ((u & (width - 1u)) != 0u) return;
+#endif
+    uint32_t *c = cntg + ((u / width) * (nl + 1u) + l) * 4u;
+#ifdef MCRG_HOST_EMULATION
+    c[0] += d0;  // the emulation visits every lane of the team in turn
+    c[1] += d1;
+    c[2] += d2;
+    c[3] += d3;
+#else
+    c[0] = d0;
+    c[1] = d1;
+    c[2] = d2;
+    c[3] = d3;
+#endif
+}
+
+// Wave-uniform value: when a lattice's team covers whole waves its key and
+// thresholds are the same in every lane and can live in scalar registers.
+template <bool UNIFORM>
+__device__ __forceinline__ uint32_t uni(uint32_t v)
+{
+#ifdef MCRG_HOST_EMULATION
+    return v;
+#else
+    return UNIFORM ? (uint32_t)__builtin_amdgcn_readfirstlane((int)v) : v;
+#endif
+}
+
+// Half-sweep of colour c for the team's lattice.
+template <bool UNIFORM>
+__device__ __forceinline__ void sweep_colour(const uint32_t *meta, uint32_t *spins, uint32_t L, uint32_t W,
+                                             uint32_t G, const Team &tm, uint32_t it, uint32_t c)
+{
+    for (uint32_t g = tm.g0; g < G; g += tm.gs) {
+        const uint32_t *m = meta + g * kMetaWords;
+        if (!in_run(m, it)) continue;
+        uint32_t *own = spins + g * 2u * W + c * W;
+        const uint32_t *oth = spins + g * 2u * W + (1u - c) * W;
+        Accept acc;
+        acc.thr0 = uni<UNIFORM>(m[kMetaThr0]);
+        acc.thr1 = uni<UNIFORM>(m[kMetaThr1]);
+        acc.always0 = uni<UNIFORM>(m[kMetaAlw0]);
+        acc.always1 = uni<UNIFORM>(m[kMetaAlw1]);
+        const uint32_t k0 = uni<UNIFORM>(m[kMetaK0]), k1 = uni<UNIFORM>(m[kMetaK1]);
+        const uint32_t t = uni<UNIFORM>(m[kMetaT]) + it;
+        for (uint32_t u = tm.u0; u < tm.T; u += tm.us) {
+            for (uint32_t w = u; w < W; w += tm.T) {
+                const Nbr4 n = cb_neighbours(oth, L, W, w, c);
+                own[w] ^= metropolis_word(own[w], n.nu, n.nd, n.nl, n.nr, 0xFFFFFFFFu, acc, k0, k1, w, t, c);
+            }
+        }
+    }
+}
+
+#ifndef MCRG_HOST_EMULATION
+// First level whose rows fit one per lane of the team's first wave, or 0 when
+// no level from 1 to nl does (L = 8, or L = 16 with one level).
+__device__ __forceinline__ uint32_t pyramid_level(uint32_t L, uint32_t T, uint32_t nl)
+{
+    const uint32_t lanes = T < 64u ? T : 64u;
+    for (uint32_t l = 1; l <= nl; ++l)
+        if ((L >> l) <= lanes) return l;
+    return 0u;
+}
+
+// Levels pl..nl of one lattice in registers: lane y (< 64, within the team's
+// first wave) holds row y of the current level (rows of 64 bits in two words),
+// neighbour rows come from cross-lane shuffles, blocked rows are formed from
+// lanes 2by and 2by+1, and the counts of each level are summed with DPP.
+// `width` is the shuffle segment (the team, or 64 when the team spans waves).
+__device__ __forceinline__ void pyramid(const uint32_t *lg, const uint32_t *cg, uint32_t *cntg, uint32_t L,
+                                        uint32_t pl, uint32_t nl, uint32_t lvl_off, uint32_t coin_off,
+                                        uint32_t y, uint32_t width)
+{
+    uint32_t Ll = L >> pl;
+    uint32_t lo = 0, hi = 0;
+    if (y < Ll) {
+        if (Ll == 64u) {
+            lo = lg[lvl_off + 2u * y];
+            hi = lg[lvl_off + 2u * y + 1u];
+        } else {
+            lo = rm_row_small(lg + lvl_off, Ll, y);
+        }
+    }
+    uint32_t coff = coin_off;
+    for (uint32_t l = pl; l <= nl; ++l) {
+        // ---- operators of level l
+        const uint32_t yn = (y + 1u) & (Ll - 1u);
+        const uint32_t dlo = (uint32_t)__shfl((int)lo, (int)yn, (int)width);
+        const uint32_t dhi = (uint32_t)__shfl((int)hi, (int)yn, (int)width);
+        uint32_t dnn = 0, dnnn = 0, dp = 0;
+        if (y < Ll) {
+            if (Ll == 64u) {
+                const uint32_t rlo = (lo >> 1) | (hi << 31), rhi = (hi >> 1) | (lo << 31);
+                const uint32_t glo = (dlo >> 1) | (dhi << 31), ghi = (dhi >> 1) | (dlo << 31);
+                dnn = (uint32_t)(popc(lo ^ rlo) + popc(hi ^ rhi) + popc(lo ^ dlo) + popc(hi ^ dhi));
+                dnnn = (uint32_t)(popc(lo ^ glo) + popc(hi ^ ghi) + popc(rlo ^ dlo) + popc(rhi ^ dhi));
+                dp = (uint32_t)(popc(lo ^ rlo ^ dlo ^ glo) + popc(hi ^ rhi ^ dhi ^ ghi));
+            } else {
+                const uint32_t r = Ll == 32u ? ((lo >> 1) | (lo << 31)) : rot_right_small(lo, Ll);
+                const uint32_t g = Ll == 32u ? ((dlo >> 1) | (dlo << 31)) : rot_right_small(dlo, Ll);
+                dnn = (uint32_t)(popc(lo ^ r) + popc(lo ^ dlo));
+                dnnn = (uint32_t)(popc(lo ^ g) + popc(r ^ dlo));
+                dp = (uint32_t)popc(lo ^ r ^ dlo ^ g);
+            }
+        }
+        dnn = group_sum(dnn, width);
+        dnnn = group_sum(dnnn, width);
+        dp = group_sum(dp, width);
+        if (y == 0u) {
+            uint32_t *c = cntg + l * 4u;
+            c[0] = dnn;
+            c[1] = dnnn;
+            c[2] = dp;
+            c[3] = 0u;
+        }
+        if (l == nl) break;
+        // ---- blocking into level l + 1: lane by takes rows 2by and 2by + 1
+        const uint32_t Lb = Ll >> 1;
+        const uint32_t s0 = (2u * y) & (Ll - 1u), s1 = (2u * y + 1u) & (Ll - 1u);
+        const uint32_t r0lo = (uint32_t)__shfl((int)lo, (int)s0, (int)width);
+        const uint32_t r1lo = (uint32_t)__shfl((int)lo, (int)s1, (int)width);
+        const uint32_t r0hi = (uint32_t)__shfl((int)hi, (int)s0, (int)width);
+        const uint32_t r1hi = (uint32_t)__shfl((int)hi, (int)s1, (int)width);
+        uint32_t nlo = 0;
+        if (y < Lb) {
+            const uint32_t bm = y * Lb;
+            const uint32_t coin = cg[coff + (bm >> 5)] >> (bm & 31u);
+            if (Ll == 64u)
+                nlo = block16(r0lo, r1lo, coin & 0xFFFFu) | (block16(r0hi, r1hi, coin >> 16) << 16);
+            else
+                nlo = block16(r0lo, r1lo, coin) & ((1u << Lb) - 1u);
+        }
+        coff += level_words(L, l + 1u);
+        lo = nlo;
+        hi = 0u;
+        Ll = Lb;
+    }
+}
+#endif
+
 __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
-                         PackedLattice *__restrict__ lats, uint32_t n_chunk, uint32_t *lds,
-                         uint32_t tid, uint32_t nt)
+                         PackedLattice *__restrict__ lats, uint32_t n_chunk, uint32_t *lds, Team tm)
 {
     const uint32_t L = (uint32_t)sl.L, G = (uint32_t)sl.G, nl = (uint32_t)sl.n_levels;
-    const SlotLayout lay = slot_layout(L, G, nl);
-    const uint32_t W = lay.W, rs = record_size(nl);
-    const uint32_t cstride = (nl + 1u) * 4u;
+    const uint32_t T = tm.T;
+    const SlotLayout lay = slot_layout(L, G, nl, T);
+    const uint32_t W = lay.W, rs = record_size(nl), nw = lay.team_waves;
+    const uint32_t cstride = nw * (nl + 1u) * 4u;
     uint32_t *meta = lds + lay.meta;
     uint32_t *spins = lds + lay.spins;
     uint32_t *lvl = lds + lay.lvl;
+    uint32_t *coins = lds + lay.coin;
     uint32_t *cnt = lds + lay.cnt;
     int64_t *rec = reinterpret_cast<int64_t *>(lds + lay.rec);
+    int64_t *sv = reinterpret_cast<int64_t *>(lds + lay.sv);
 
-    for (uint32_t g = tid; g < G; g += nt) {
-        uint32_t *m = meta + g * kMetaWords;
-        const int32_t li = slot_lats[sl.first + (int32_t)g];
-        if (li >= 0) {
-            const PackedLattice &P = lats[li];
-            m[kMetaIdx] = (uint32_t)li;
-            m[kMetaT] = P.t;
-            m[kMetaTherm] = P.therm_left;
-            m[kMetaMeas] = P.meas_left;
-            m[kMetaNMeas] = P.n_measured;
-            m[kMetaSpb] = (uint32_t)P.sweeps_per_bin;
-            m[kMetaK0] = P.k0;
-            m[kMetaK1] = P.k1;
-            m[kMetaThr0] = P.acc.thr0;
-            m[kMetaThr1] = P.acc.thr1;
-            m[kMetaAlw0] = P.acc.always0;
-            m[kMetaAlw1] = P.acc.always1;
-        } else {
-            m[kMetaIdx] = 0xFFFFFFFFu;
-            m[kMetaTherm] = 0u;
-            m[kMetaMeas] = 0u;
+    // ---- load: meta, spins, zero partial records
+    for (uint32_t g = tm.g0; g < G; g += tm.gs) {
+        for (uint32_t u = tm.u0; u < 1u; u += tm.us) {
+            uint32_t *m = meta + g * kMetaWords;
+            const int32_t li = slot_lats[sl.first + (int32_t)g];
+            if (li >= 0) {
+                const PackedLattice &P = lats[li];
+                m[kMetaIdx] = (uint32_t)li;
+                m[kMetaT] = P.t;
+                m[kMetaTherm] = P.therm_left;
+                m[kMetaMeas] = P.meas_left;
+                m[kMetaNMeas] = P.n_measured;
+                m[kMetaSpb] = (uint32_t)P.sweeps_per_bin;
+                m[kMetaK0] = P.k0;
+                m[kMetaK1] = P.k1;
+                m[kMetaThr0] = P.acc.thr0;
+                m[kMetaThr1] = P.acc.thr1;
+                m[kMetaAlw0] = P.acc.always0;
+                m[kMetaAlw1] = P.acc.always1;
+            } else {
+                m[kMetaIdx] = 0xFFFFFFFFu;
+                m[kMetaTherm] = 0u;
+                m[kMetaMeas] = 0u;
+            }
         }
     }
-    MCRG_SYNC();
-    for (uint32_t i = tid; i < G * 2u * W; i += nt) {
-        const uint32_t g = i / (2u * W), j = i % (2u * W);
+    block_sync();
+    for (uint32_t g = tm.g0; g < G; g += tm.gs) {
         const uint32_t li = meta[g * kMetaWords + kMetaIdx];
-        spins[i] = li != 0xFFFFFFFFu ? lats[li].spins[j] : 0u;
+        for (uint32_t u = tm.u0; u < T; u += tm.us) {
+            for (uint32_t j = u; j < 2u * W; j += T)
+                spins[g * 2u * W + j] = li != 0xFFFFFFFFu ? lats[li].spins[j] : 0u;
+            for (uint32_t k = u; k < rs; k += T) rec[g * rs + k] = 0;
+        }
     }
-    for (uint32_t i = tid; i < G * rs; i += nt) rec[i] = 0;
-    MCRG_SYNC();
+    block_sync();
 
     for (uint32_t it = 0; it < n_chunk; ++it) {
-        int running = 0, measuring = 0;
-        for (uint32_t g = tid; g < G; g += nt) {
-            running |= in_run(meta + g * kMetaWords, it) ? 1 : 0;
-            measuring |= in_meas(meta + g * kMetaWords, it) ? 1 : 0;
+        bool running = false, measuring = false;
+        for (uint32_t g = tm.g0; g < G; g += tm.gs) {
+            running = running || in_run(meta + g * kMetaWords, it);
+            measuring = measuring || in_meas(meta + g * kMetaWords, it);
         }
-        if (!MCRG_SYNC_OR(running)) break;
+        if (!team_any(tm, running)) break;
 
         // ---- sweep: colour 0, then colour 1
         for (uint32_t c = 0; c < 2u; ++c) {
-            for (uint32_t i = tid; i < G * W; i += nt) {
-                const uint32_t g = i / W, w = i % W;
-                const uint32_t *m = meta + g * kMetaWords;
-                if (!in_run(m, it)) continue;
-                uint32_t *base = spins + g * 2u * W;
-                uint32_t *own = base + c * W;
-                const uint32_t *oth = base + (1u - c) * W;
-                const Nbr4 n = cb_neighbours(oth, L, W, w, c);
-                Accept acc;
-                acc.thr0 = m[kMetaThr0];
-                acc.thr1 = m[kMetaThr1];
-                acc.always0 = m[kMetaAlw0];
-                acc.always1 = m[kMetaAlw1];
-                const uint32_t flip = metropolis_word(own[w], n.nu, n.nd, n.nl, n.nr, 0xFFFFFFFFu, acc,
-                                                      m[kMetaK0], m[kMetaK1], w, m[kMetaT] + it, c);
-                own[w] ^= flip;
-            }
-            MCRG_SYNC();
+            if (T >= 64u)
+                sweep_colour<true>(meta, spins, L, W, G, tm, it, c);
+            else
+                sweep_colour<false>(meta, spins, L, W, G, tm, it, c);
+            team_sync(tm);
         }
-        if (!MCRG_SYNC_OR(measuring)) continue;
+        if (!team_any(tm, measuring)) continue;
 
-        // ---- level 0: operators and blocking into level 1
-        for (uint32_t i = tid; i < G * cstride; i += nt) cnt[i] = 0u;
-        MCRG_SYNC();
-        {
-            const uint32_t per = 2u * W, n_items = G * per;
-            const uint32_t rounds = ceil_div(n_items, nt);
-            for (uint32_t r = 0; r < rounds; ++r) {
-                const uint32_t i = tid + r * nt;
+        // ---- coins of every blocking level: call c of the flattened list gives
+        //      coin words 4q..4q+3 of its level
+        for (uint32_t g = tm.g0; g < G; g += tm.gs) {
+            const uint32_t *m = meta + g * kMetaWords;
+            if (!in_meas(m, it)) continue;
+            const uint32_t t = m[kMetaT] + it;
+            for (uint32_t u = tm.u0; u < T; u += tm.us) {
+                for (uint32_t c = u; c < lay.coin_calls; c += T) {
+                    uint32_t l = 0, q = c, off = 0;
+                    for (; l + 1u < nl; ++l) {
+                        const uint32_t calls = (level_words(L, l + 1u) + 3u) / 4u;
+                        if (q < calls) break;
+                        q -= calls;
+                        off += level_words(L, l + 1u);
+                    }
+                    const uint32_t n_words = level_words(L, l + 1u);
+                    const u32x4 r = philox(q, t, 0u, (kTagTie << 28) | (l << 16), m[kMetaK0], m[kMetaK1]);
+                    uint32_t *cw = coins + g * lay.lvl_words + off + 4u * q;
+                    cw[0] = r.v0;
+                    if (4u * q + 1u < n_words) cw[1] = r.v1;
+                    if (4u * q + 2u < n_words) cw[2] = r.v2;
+                    if (4u * q + 3u < n_words) cw[3] = r.v3;
+                }
+            }
+        }
+        team_sync(tm);
+
+        // ---- level 0: operator counts and blocking into level 1
+        for (uint32_t g = tm.g0; g < G; g += tm.gs) {
+            const uint32_t *m = meta + g * kMetaWords;
+            const bool meas = in_meas(m, it);
+            const uint32_t *c0 = spins + g * 2u * W, *c1 = c0 + W;
+            for (uint32_t u = tm.u0; u < T; u += tm.us) {
                 uint32_t dnn = 0, dnnn = 0, dp = 0, nd = 0;
-                if (i < n_items) {
-                    const uint32_t g = i / per, j = i % per, c = j / W, w = j % W;
-                    if (in_meas(meta + g * kMetaWords, it)) {
-                        const uint32_t *base = spins + g * 2u * W;
-                        cb_ops_item(base + c * W, base + (1u - c) * W, L, W, w, c, dnn, dnnn, dp, nd);
+                if (meas) {
+                    for (uint32_t w = u; w < W; w += T) {
+                        cb_ops_item(c0, c1, L, W, w, 0u, dnn, dnnn, dp, nd);
+                        cb_ops_item(c1, c0, L, W, w, 1u, dnn, dnnn, dp, nd);
+                    }
+                    if (nl > 0u) {
+                        const uint32_t nb = cb_block_items(L);
+                        uint32_t *l1 = lvl + g * lay.lvl_words;
+                        const uint32_t *cw = coins + g * lay.lvl_words;
+                        for (uint32_t j = u; j < nb; j += T) l1[j] = cb_block_item(c0, c1, L, W, j, cw[j]);
                     }
                 }
-                uint32_t *c0 = cnt;
-                add_count(c0 + 0, cstride, i, per, n_items, dnn, tid);
-                add_count(c0 + 1, cstride, i, per, n_items, dnnn, tid);
-                add_count(c0 + 2, cstride, i, per, n_items, dp, tid);
-                add_count(c0 + 3, cstride, i, per, n_items, nd, tid);
-            }
-            const uint32_t nb = nl > 0u ? cb_block_items(L) : 0u;  // no level-1 scratch without levels
-            for (uint32_t i = tid; i < G * nb; i += nt) {
-                const uint32_t g = i / nb, j = i % nb;
-                const uint32_t *m = meta + g * kMetaWords;
-                if (!in_meas(m, it)) continue;
-                const uint32_t *base = spins + g * 2u * W;
-                const uint32_t coin = tie_coin_word(j, m[kMetaT] + it, 0u, m[kMetaK0], m[kMetaK1]);
-                lvl[g * lay.lvl_words + j] = cb_block_item(base, base + W, L, W, j, coin);
+                store_counts(cnt + g * cstride, tm, nl, 0u, u, dnn, dnnn, dp, nd);
             }
         }
-        MCRG_SYNC();
+        team_sync(tm);
 
-        // ---- levels 1..nl
+        // ---- levels 1..nl: spread over the team while a level has more rows
+        //      than the team's first wave has lanes, then the register pyramid
+#ifdef MCRG_HOST_EMULATION
+        const uint32_t pl = 0u;
+#else
+        const uint32_t pl = pyramid_level(L, T, nl);
+#endif
+        const uint32_t last_spread = pl ? pl - 1u : nl;
         uint32_t off = 0;
-        for (uint32_t l = 1; l <= nl; ++l) {
+        for (uint32_t l = 1; l <= last_spread; ++l) {
             const uint32_t Ll = L >> l;
-            const uint32_t per = rm_ops_items(Ll), n_items = G * per;
-            const uint32_t rounds = ceil_div(n_items, nt);
-            for (uint32_t r = 0; r < rounds; ++r) {
-                const uint32_t i = tid + r * nt;
-                uint32_t dnn = 0, dnnn = 0, dp = 0;
-                if (i < n_items) {
-                    const uint32_t g = i / per, j = i % per;
-                    if (in_meas(meta + g * kMetaWords, it))
-                        rm_ops_item(lvl + g * lay.lvl_words + off, Ll, j, dnn, dnnn, dp);
+            const uint32_t nxt = off + level_words(L, l);
+            for (uint32_t g = tm.g0; g < G; g += tm.gs) {
+                const bool meas = in_meas(meta + g * kMetaWords, it);
+                uint32_t *lg = lvl + g * lay.lvl_words;
+                const uint32_t *cw = coins + g * lay.lvl_words + nxt;
+                for (uint32_t u = tm.u0; u < T; u += tm.us) {
+                    uint32_t dnn = 0, dnnn = 0, dp = 0;
+                    if (meas) {
+                        const uint32_t n_ops = rm_ops_items(Ll);
+                        for (uint32_t i = u; i < n_ops; i += T) rm_ops_item(lg + off, Ll, i, dnn, dnnn, dp);
+                        if (l < nl) {
+                            const uint32_t nb = rm_block_items(Ll);
+                            for (uint32_t j = u; j < nb; j += T) lg[nxt + j] = rm_block_item(lg + off, Ll, j, cw[j]);
+                        }
+                    }
+                    store_counts(cnt + g * cstride, tm, nl, l, u, dnn, dnnn, dp, 0u);
                 }
-                uint32_t *cl = cnt + l * 4u;
-                add_count(cl + 0, cstride, i, per, n_items, dnn, tid);
-                add_count(cl + 1, cstride, i, per, n_items, dnnn, tid);
-                add_count(cl + 2, cstride, i, per, n_items, dp, tid);
             }
-            if (l < nl) {
-                const uint32_t nb = rm_block_items(Ll);
-                const uint32_t nxt = off + level_words(L, l);
-                for (uint32_t i = tid; i < G * nb; i += nt) {
-                    const uint32_t g = i / nb, j = i % nb;
-                    const uint32_t *m = meta + g * kMetaWords;
-                    if (!in_meas(m, it)) continue;
-                    const uint32_t coin = tie_coin_word(j, m[kMetaT] + it, l, m[kMetaK0], m[kMetaK1]);
-                    uint32_t *lg = lvl + g * lay.lvl_words;
-                    lg[nxt + j] = rm_block_item(lg + off, Ll, j, coin);
-                }
-                off = nxt;
-            }
-            MCRG_SYNC();
+            off = nxt;
+            team_sync(tm);
         }
 
-        // ---- the sample's sums and products into the partial records
-        for (uint32_t i = tid; i < G * rs; i += nt) {
-            const uint32_t g = i / rs, k = i % rs;
+#ifndef MCRG_HOST_EMULATION
+        if (pl) {
+            // `off` is now the RM offset of level pl; its coins start one level later
+            const uint32_t coin_off = off + level_words(L, pl);
+            for (uint32_t g = tm.g0; g < G; g += tm.gs) {
+                const bool meas = in_meas(meta + g * kMetaWords, it);
+                for (uint32_t u = tm.u0; u < T; u += tm.us) {
+                    uint32_t *cntg = cnt + g * cstride + (u >> 6) * (nl + 1u) * 4u;
+                    if (u < 64u) {
+                        if (meas)
+                            pyramid(lvl + g * lay.lvl_words, coins + g * lay.lvl_words, cntg, L, pl, nl, off,
+                                    coin_off, u, T < 64u ? T : 64u);
+                    } else if ((u & 63u) == 0u) {
+                        for (uint32_t l = pl; l <= nl; ++l)
+                            for (uint32_t a = 0; a < 4u; ++a) cntg[l * 4u + a] = 0u;
+                    }
+                }
+            }
+            team_sync(tm);
+        }
+#endif
+
+        // ---- operator values, then the sample's sums and products into the
+        //      partial records
+        for (uint32_t g = tm.g0; g < G; g += tm.gs) {
+            if (!in_meas(meta + g * kMetaWords, it)) continue;
+            for (uint32_t u = tm.u0; u < T; u += tm.us)
+                for (uint32_t i = u; i < lay.n_sv; i += T) sv[g * lay.n_sv + i] = sv_value(cnt + g * cstride, nw, nl, L, i);
+        }
+        team_sync(tm);
+        for (uint32_t g = tm.g0; g < G; g += tm.gs) {
             const uint32_t *m = meta + g * kMetaWords;
             if (!in_meas(m, it)) continue;
             const uint32_t q = m[kMetaNMeas] + (it - m[kMetaTherm]);
             const uint32_t spb = m[kMetaSpb];
-            if (q % spb == 0u && q > m[kMetaNMeas]) {
-                lats[m[kMetaIdx]].records[(size_t)((q - 1u) / spb) * rs + k] += rec[i];
-                rec[i] = 0;
+            const bool new_bin = q % spb == 0u && q > m[kMetaNMeas];
+            const int64_t *svg = sv + g * lay.n_sv;
+            for (uint32_t u = tm.u0; u < T; u += tm.us) {
+                for (uint32_t k = u; k < rs; k += T) {
+                    int64_t &r = rec[g * rs + k];
+                    if (new_bin) {
+                        lats[m[kMetaIdx]].records[(size_t)((q - 1u) / spb) * rs + k] += r;
+                        r = 0;
+                    }
+                    uint32_t x, y;
+                    record_pair(nl, k, x, y);
+                    r += svg[x] * svg[y];
+                }
             }
-            rec[i] += record_term(cnt + g * cstride, L, nl, k);
         }
-        MCRG_SYNC();
+        team_sync(tm);
+#ifdef MCRG_HOST_EMULATION
+        for (uint32_t i = 0; i < G * cstride; ++i) cnt[i] = 0u;
+#endif
     }
+    block_sync();
 
     // ---- write back: partial records, spins, counters
-    for (uint32_t i = tid; i < G * rs; i += nt) {
-        const uint32_t g = i / rs, k = i % rs;
-        const uint32_t *m = meta + g * kMetaWords;
-        if (m[kMetaIdx] == 0xFFFFFFFFu) continue;
-        const uint32_t total = m[kMetaTherm] + m[kMetaMeas];
-        const uint32_t swept = total < n_chunk ? total : n_chunk;
-        const uint32_t measured = swept > m[kMetaTherm] ? swept - m[kMetaTherm] : 0u;
-        if (measured == 0u) continue;
-        const uint32_t q_last = m[kMetaNMeas] + measured - 1u;
-        lats[m[kMetaIdx]].records[(size_t)(q_last / m[kMetaSpb]) * rs + k] += rec[i];
-    }
-    for (uint32_t i = tid; i < G * 2u * W; i += nt) {
-        const uint32_t g = i / (2u * W), j = i % (2u * W);
-        const uint32_t li = meta[g * kMetaWords + kMetaIdx];
-        if (li != 0xFFFFFFFFu) lats[li].spins[j] = spins[i];
-    }
-    for (uint32_t g = tid; g < G; g += nt) {
+    for (uint32_t g = tm.g0; g < G; g += tm.gs) {
         const uint32_t *m = meta + g * kMetaWords;
         if (m[kMetaIdx] == 0xFFFFFFFFu) continue;
         PackedLattice &P = lats[m[kMetaIdx]];
         const uint32_t total = m[kMetaTherm] + m[kMetaMeas];
         const uint32_t swept = total < n_chunk ? total : n_chunk;
         const uint32_t measured = swept > m[kMetaTherm] ? swept - m[kMetaTherm] : 0u;
-        P.t = m[kMetaT] + swept;
-        P.n_measured = m[kMetaNMeas] + measured;
-        P.therm_left = m[kMetaTherm] - (swept - measured);
-        P.meas_left = m[kMetaMeas] - measured;
+        for (uint32_t u = tm.u0; u < T; u += tm.us) {
+            if (measured > 0u) {
+                const uint32_t q_last = m[kMetaNMeas] + measured - 1u;
+                for (uint32_t k = u; k < rs; k += T)
+                    P.records[(size_t)(q_last / m[kMetaSpb]) * rs + k] += rec[g * rs + k];
+            }
+            for (uint32_t j = u; j < 2u * W; j += T) P.spins[j] = spins[g * 2u * W + j];
+            if (u == 0u) {
+                P.t = m[kMetaT] + swept;
+                P.n_measured = m[kMetaNMeas] + measured;
+                P.therm_left = m[kMetaTherm] - (swept - measured);
+                P.meas_left = m[kMetaMeas] - measured;
+            }
+        }
     }
-    MCRG_SYNC();
+    block_sync();
 }
 
 }  // namespace mcrg

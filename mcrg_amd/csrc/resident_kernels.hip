@@ -47,7 +47,16 @@ __global__ __launch_bounds__(NT) void k_resident(const Slot *__restrict__ slots,
         const uint32_t idx = s_idx;
         __syncthreads();
         if (idx >= n_slots) break;
-        run_slot(slots[idx], slot_lats, lats, n_chunk, lds, threadIdx.x, NT);
+        const Slot sl = slots[idx];
+        Team tm;
+        tm.T = (uint32_t)NT / (uint32_t)sl.G;
+        tm.g0 = threadIdx.x / tm.T;
+        tm.gs = (uint32_t)sl.G;
+        tm.u0 = threadIdx.x % tm.T;
+        tm.us = tm.T;
+        tm.lane = threadIdx.x;
+        tm.block = tm.T > 64u;
+        run_slot(sl, slot_lats, lats, n_chunk, lds, tm);
     }
 }
 

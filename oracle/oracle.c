@@ -81,20 +81,24 @@ void orc_init_spins(int L, uint64_t seed, int8_t *s)
 
 /* One half-sweep over colour `color`.  This replaces one Wolff cluster
  * (IsingModel::sample_new_configuration, src/ising.cpp:87-149); neighbours follow
- * Lattice::nearest_neighbors (src/lattice.cpp:124-136). */
+ * Lattice::nearest_neighbors (src/lattice.cpp:124-136).
+ *
+ * Random draw of site i of word w: the 4 top bits of U_i are bit i of the slices
+ * out[0..3] of Philox(w, t, 0, tag) (out[0] gives bit 31).  When they equal the
+ * threshold's 4 top bits the comparison is still open; the open sites of the
+ * word, ranked by increasing i (rank rho = 0, 1, ...), take their 28 low bits
+ * from word rho & 3 of Philox(w, t, 1 + (rho >> 2), tag) (its top 28 bits).
+ * U_i is uniform on [0, 2^32) and independent across sites. */
 void orc_half_sweep(int L, int8_t *s, uint64_t seed, const uint64_t thr[2], uint32_t t, int color)
 {
     const int half = L / 2;
     const int n_sites = L * L / 2;
     const int n_words = (n_sites + 31) / 32;
+    const uint32_t tag = ((uint32_t)ORC_TAG_SWEEP << 28) | ((uint32_t)color << 24);
     for (int w = 0; w < n_words; ++w) {
-        uint32_t r[32];
-        for (int j = 0; j < 8; ++j) {
-            uint32_t out[4];
-            philox_seeded(seed, (uint32_t)w, t, (uint32_t)j,
-                          ((uint32_t)ORC_TAG_SWEEP << 28) | ((uint32_t)color << 24), out);
-            for (int q = 0; q < 4; ++q) r[4 * j + q] = out[q];
-        }
+        uint32_t first[4], refine[4];
+        philox_seeded(seed, (uint32_t)w, t, 0u, tag, first);
+        int rank = 0;
         for (int i = 0; i < 32; ++i) {
             int m = 32 * w + i;
             if (m >= n_sites) break;
@@ -110,9 +114,22 @@ void orc_half_sweep(int L, int8_t *s, uint64_t seed, const uint64_t thr[2], uint
                 flip = 1;
             } else {
                 uint64_t T = thr[e / 2 - 1];
-                uint32_t U = 0;
-                for (int kk = 0; kk < 32; ++kk) U |= ((r[kk] >> i) & 1u) << (31 - kk);
-                flip = ((uint64_t)U < T);
+                if (T >= 4294967296ull) {
+                    flip = 1;
+                } else {
+                    uint32_t top = 0;
+                    for (int kk = 0; kk < 4; ++kk) top |= ((first[kk] >> i) & 1u) << (3 - kk);
+                    uint32_t ttop = (uint32_t)(T >> 28);
+                    if (top != ttop) {
+                        flip = top < ttop;
+                    } else {
+                        if ((rank & 3) == 0)
+                            philox_seeded(seed, (uint32_t)w, t, 1u + (uint32_t)(rank >> 2), tag, refine);
+                        uint32_t U = (top << 28) | (refine[rank & 3] >> 4);
+                        flip = (uint64_t)U < T;
+                        ++rank;
+                    }
+                }
             }
             if (flip) s[y * L + x] = (int8_t)(-si);
         }

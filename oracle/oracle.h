@@ -30,13 +30,16 @@
  *            Word w = m >> 5 holds 32 sites, site bit i = m & 31.
  *  Philox    Philox4x32-10 (Salmon et al., SC'11), key = (lo32(seed), hi32(seed)),
  *            counter (x0, x1, x2, x3), output out[0..3].
- *  Sweep t   half-sweep colour 0, then colour 1.  For word w of colour c the 32
- *            random "slices" r_0..r_31 are r_k = out[k & 3] of
- *            Philox(x0 = w, x1 = t, x2 = k >> 2, x3 = TAG_SWEEP << 28 | c << 24).
- *            Site i of the word draws U_i = sum_k ((r_k >> i) & 1) << (31 - k).
- *            With h = sum of the 4 neighbours and e = s*h:  e <= 0 -> flip;
- *            e = 2 -> flip iff U < thr[0];  e = 4 -> flip iff U < thr[1];
- *            thr[q] = floor(exp(-4 J (q+1)) * 2^32) (a value of 2^32 means always).
+ *  Sweep t   half-sweep colour 0, then colour 1.  For word w of colour c let
+ *            tag = TAG_SWEEP << 28 | c << 24.  Site i of the word draws a 32-bit
+ *            U_i: its 4 top bits are bit i of out[0..3] of Philox(w, t, 0, tag)
+ *            (out[0] -> bit 31).  With h = sum of the 4 neighbours, e = s*h:
+ *            e <= 0 -> flip; e = 2 -> flip iff U < thr[0]; e = 4 -> iff U < thr[1],
+ *            thr[q] = floor(exp(-4 J (q+1)) * 2^32) (2^32 means always).  When the
+ *            4 top bits of U_i equal those of thr the site is "open": the open
+ *            sites of the word, ranked rho = 0, 1, ... by increasing i, take the
+ *            top 28 bits of word rho & 3 of Philox(w, t, 1 + (rho >> 2), tag) as
+ *            the 28 low bits of U_i.
  *  Init      site m = y*L + x gets bit (out[(m>>5)&3] >> (m&31)) & 1 of
  *            Philox(x0 = m >> 7, 0, 0, TAG_INIT << 28); s = 1 - 2 bit.
  *  Blocking  b = 2 majority rule from level l (size Ll) to level l+1 at sweep t:
