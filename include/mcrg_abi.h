@@ -54,14 +54,16 @@ const char *mcrg_last_error(void);
 mcrg_status mcrg_thresholds(double coupling, uint64_t thr[2]);
 
 /* int64 entries in one per-bin record for n_levels blockings:
- * 21 n_levels + 16.  Layout (l = 0..n_levels, a, b in {NN, NNN, P}):
+ * 22 n_levels + 16.  Layout (l = 0..n_levels, a, b in {NN, NNN, P}):
  *   [l*3 + a]                                 sum S_l[a]
  *   [3(nl+1) + l*9 + b*3 + a]                 sum S_l[a] S_l[b]
  *   [12(nl+1) + (l-1)*9 + b*3 + a], l >= 1    sum S_l[a] S_{l-1}[b]
- *   [12(nl+1) + 9nl + 0..3]                   sum M, sum |M|, sum M^2, samples
+ *   [12(nl+1) + 9nl + l - 1], l >= 1          sum NN_l NN_0
+ *   [12(nl+1) + 10nl + 0..3]                  sum M, sum |M|, sum M^2, samples
  * These are the accumulators S_avg_loc, Sb_Sb_avg_loc, Sb_S_avg_loc of
  * calc_critical_exponent (src/mcrg.cpp:59-66, flattened as
- * src/definitions.cpp:9-19), kept exactly in int64 per bin. */
+ * src/definitions.cpp:9-19) and SLb_SL_avg_loc of approx_critical_point
+ * (src/mcrg.cpp:262-263), kept exactly in int64 per bin. */
 int32_t mcrg_record_size(int32_t n_levels);
 
 /* A context owns one device.  Replaces the MPI_Init / MPI_COMM_WORLD setup of
@@ -83,7 +85,7 @@ mcrg_status mcrg_ctx_device_info(mcrg_ctx ctx, mcrg_device_info *out);
 
 /* ------------------------------------------------------------------ chains */
 typedef struct {
-    int32_t L;              /* side, power of two, 4 <= L <= 16384          */
+    int32_t L;              /* side, power of two, 2 <= L <= 16384          */
     double coupling;        /* J = |K| >= 0                                  */
     uint64_t seed;          /* Philox key of this chain                      */
     int32_t n_levels;       /* blockings per sample, 0 <= n <= log2(L) - 1   */
@@ -176,6 +178,47 @@ mcrg_status mcrg_batch_get_records(mcrg_batch b, int32_t job, int64_t *host, int
 /* Spins of job j as +-1 int8, row-major, n = L*L. */
 mcrg_status mcrg_batch_get_spins(mcrg_batch b, int32_t job, int8_t *host, int64_t n);
 mcrg_status mcrg_batch_get_state(mcrg_batch b, int32_t job, uint32_t *sweep, uint32_t *n_measured);
+
+/* ------------------------------------------------------------------ FP64 solve */
+/* Host arithmetic on records; no device work.  The connected correlators are
+ * formed exactly from the int64 sums in 128-bit integers, then in FP64.
+ *
+ * Linearised RG matrix between levels `level` and level + 1 from one record
+ * (e.g. the sum of all bins), restricted to the first n_ops operators (2 = NN,
+ * NNN as the reference; 3 adds the plaquette):
+ *   dSb_dK  = <S_{l+1} S_l^T> - <S_{l+1}><S_l>^T
+ *   dSb_dKb = <S_{l+1} S_{l+1}^T> - <S_{l+1}><S_{l+1}>^T
+ *   T = dSb_dKb^-1 dSb_dK                      (src/mcrg.cpp:114-123)
+ * T is row-major, n_ops x n_ops.  Single- and double-counted operators give the
+ * same T up to a similarity transform, hence the same eigenvalues. */
+mcrg_status mcrg_rg_matrix(const int64_t *record, int32_t n_levels, int32_t level, int32_t n_ops,
+                           double *T);
+/* Largest real part of the eigenvalues of a real n x n matrix, n <= 3 (the
+ * reference's EigenSolver + max of real parts, src/mcrg.cpp:126-130). */
+double mcrg_leading_eigenvalue(int32_t n, const double *T);
+
+typedef struct {
+    double lambda;     /* leading eigenvalue of T at this level           */
+    double yt;         /* ln(lambda) / ln 2                               */
+    double nu;         /* ln 2 / ln(lambda) = 1 / yt (src/mcrg.cpp:131)   */
+    double yt_err;     /* delete-one-bin jackknife errors (0 for 1 bin)   */
+    double nu_err;
+    int32_t n_jackknife;
+} mcrg_rg_result;
+
+/* Exponent at `level` from n_bins per-bin records (n_bins * record_size). */
+mcrg_status mcrg_rg_exponent(const int64_t *records, int32_t n_bins, int32_t n_levels, int32_t level,
+                             int32_t n_ops, mcrg_rg_result *out);
+
+/* One matching step of the critical-point locator, approx_critical_point
+ * (src/mcrg.cpp:191-310): `rec_large` is the record of an L lattice, `rec_small`
+ * that of an L/2 lattice at the same coupling; at `level` n the L lattice blocked
+ * n+1 times is matched with the L/2 lattice blocked n times through the NN
+ * operator.  *dK is the reference's step (double-counted NN in numerator and
+ * covariances, src/mcrg.cpp:293-298): the reference's new estimate is K + dK in
+ * its sign convention (K = -J), i.e. J_new = J - dK. */
+mcrg_status mcrg_critical_point_step(const int64_t *rec_large, int32_t nl_large, const int64_t *rec_small,
+                                     int32_t nl_small, int32_t level, double *dK);
 
 /* ------------------------------------------------------------------ timing */
 /* Kernel-duration statistics measured with HIP events on the stream that

@@ -52,6 +52,11 @@ class BatchOpts(ctypes.Structure):
                 ("max_group", ctypes.c_int32)]
 
 
+class RGResult(ctypes.Structure):
+    _fields_ = [("lambda_", ctypes.c_double), ("yt", ctypes.c_double), ("nu", ctypes.c_double),
+                ("yt_err", ctypes.c_double), ("nu_err", ctypes.c_double), ("n_jackknife", ctypes.c_int32)]
+
+
 class DeviceInfo(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 64), ("compute_units", ctypes.c_int32), ("clock_khz", ctypes.c_int32),
                 ("lds_per_block", ctypes.c_int32), ("warp_size", ctypes.c_int32),
@@ -89,6 +94,11 @@ _SIGS = {
     "mcrg_batch_get_records": (_c.c_int32, [_p, _c.c_int32, _p, _c.c_int64]),
     "mcrg_batch_get_spins": (_c.c_int32, [_p, _c.c_int32, _p, _c.c_int64]),
     "mcrg_batch_get_state": (_c.c_int32, [_p, _c.c_int32, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint32)]),
+    "mcrg_rg_matrix": (_c.c_int32, [_p, _c.c_int32, _c.c_int32, _c.c_int32, _c.POINTER(_c.c_double)]),
+    "mcrg_leading_eigenvalue": (_c.c_double, [_c.c_int32, _c.POINTER(_c.c_double)]),
+    "mcrg_rg_exponent": (_c.c_int32, [_p, _c.c_int32, _c.c_int32, _c.c_int32, _c.c_int32, _c.POINTER(RGResult)]),
+    "mcrg_critical_point_step": (_c.c_int32, [_p, _c.c_int32, _p, _c.c_int32, _c.c_int32,
+                                              _c.POINTER(_c.c_double)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(_lib, _name)
@@ -126,6 +136,36 @@ def record_size(n_levels):
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def rg_matrix(record, n_levels, level, n_ops):
+    r = np.ascontiguousarray(record, dtype=np.int64).reshape(-1)
+    T = (ctypes.c_double * (n_ops * n_ops))()
+    check(_lib.mcrg_rg_matrix(_ptr(r), n_levels, level, n_ops, T))
+    return np.array(list(T)).reshape(n_ops, n_ops)
+
+
+def leading_eigenvalue(T):
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    return _lib.mcrg_leading_eigenvalue(T.shape[0], T.reshape(-1).ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+
+def rg_exponent(records, n_levels, level, n_ops=2):
+    """dict(lambda, yt, nu, yt_err, nu_err, n_jackknife) from per-bin records."""
+    r = np.ascontiguousarray(records, dtype=np.int64)
+    r2 = r.reshape(-1, record_size(n_levels))
+    out = RGResult()
+    check(_lib.mcrg_rg_exponent(_ptr(r2), r2.shape[0], n_levels, level, n_ops, ctypes.byref(out)))
+    return {"lambda": out.lambda_, "yt": out.yt, "nu": out.nu, "yt_err": out.yt_err, "nu_err": out.nu_err,
+            "n_jackknife": out.n_jackknife}
+
+
+def critical_point_step(rec_large, nl_large, rec_small, nl_small, level):
+    a = np.ascontiguousarray(rec_large, dtype=np.int64).reshape(-1)
+    b = np.ascontiguousarray(rec_small, dtype=np.int64).reshape(-1)
+    d = ctypes.c_double()
+    check(_lib.mcrg_critical_point_step(_ptr(a), nl_large, _ptr(b), nl_small, level, ctypes.byref(d)))
+    return d.value
 
 
 class Context:

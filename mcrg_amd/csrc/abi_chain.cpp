@@ -181,7 +181,7 @@ mcrg_status mcrg_chain_create(mcrg_ctx ctx, const mcrg_chain_desc *desc, mcrg_ch
 {
     MCRG_REQUIRE(ctx != nullptr && desc != nullptr && out != nullptr, "null argument");
     const mcrg_chain_desc d = *desc;
-    MCRG_REQUIRE(is_pow2(d.L) && d.L >= 4 && d.L <= 16384, "L must be a power of two in [4, 16384]");
+    MCRG_REQUIRE(is_pow2(d.L) && d.L >= 2 && d.L <= 16384, "L must be a power of two in [2, 16384]");
     MCRG_REQUIRE(std::isfinite(d.coupling) && d.coupling >= 0.0, "coupling J = |K| must be >= 0");
     MCRG_REQUIRE(d.n_levels >= 0 && d.n_levels <= ilog2(d.L) - 1 && d.n_levels <= (int)kMaxLevels,
                  "n_levels must be in [0, log2(L) - 1]");

@@ -327,6 +327,7 @@ __global__ __launch_bounds__(1024) void k_levels_accumulate(const HbmLattice *__
                     if (l >= 1u) rec[off_cross(nl, l, a, b)] += S[l][a] * S[l - 1][b];
                 }
             }
+            if (l >= 1u) rec[off_nn0(nl, l)] += S[l][0] * S[0][0];
         }
         const uint32_t om = off_M(nl);
         rec[om + 0] += M;

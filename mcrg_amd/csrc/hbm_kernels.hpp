@@ -21,6 +21,8 @@ constexpr int kMaxHbmMeasureL = 2048;
 //   S[l][a]                 sum of operator a at level l           3 (nl+1)
 //   SELF[l][a][b]           sum of S[l][a] S[l][b]                 9 (nl+1)
 //   CROSS[l][a][b], l >= 1  sum of S[l][a] S[l-1][b]               9 nl
+//   NN0[l], l >= 1          sum of NN[l] NN[0] (the critical-point  nl
+//                           locator's input, src/mcrg.cpp:262-263)
 //   M, |M|, M^2 at level 0, sample count                          4
 __host__ __device__ constexpr uint32_t off_S(uint32_t l, uint32_t a) { return l * 3u + a; }
 __host__ __device__ constexpr uint32_t off_self(uint32_t nl, uint32_t l, uint32_t a, uint32_t b)
@@ -31,7 +33,8 @@ __host__ __device__ constexpr uint32_t off_cross(uint32_t nl, uint32_t l, uint32
 {
     return 12u * (nl + 1u) + (l - 1u) * 9u + b * 3u + a;
 }
-__host__ __device__ constexpr uint32_t off_M(uint32_t nl) { return 12u * (nl + 1u) + 9u * nl; }
+__host__ __device__ constexpr uint32_t off_nn0(uint32_t nl, uint32_t l) { return 12u * (nl + 1u) + 9u * nl + l - 1u; }
+__host__ __device__ constexpr uint32_t off_M(uint32_t nl) { return 12u * (nl + 1u) + 10u * nl; }
 __host__ __device__ constexpr uint32_t record_size(uint32_t nl) { return off_M(nl) + 4u; }
 
 struct HbmLattice {

@@ -109,10 +109,11 @@ __device__ __forceinline__ int64_t level_S(const uint32_t *cntg, uint32_t nw, ui
 }
 
 // Record entry k of one sample is sv[x] * sv[y] over the lattice's operator
-// values sv = (S[0][0..2], ..., S[nl][0..2], M, |M|, 1) (layout of record_size()).
+// values sv = (S[0][0..2], ..., S[nl][0..2], M, |M|, 1) (layout of record_size():
+// S, SELF, CROSS, NN0, then M, |M|, M^2 and the count).
 __device__ __forceinline__ void record_pair(uint32_t nl, uint32_t k, uint32_t &x, uint32_t &y)
 {
-    const uint32_t n_s = 3u * (nl + 1u), n_self = 9u * (nl + 1u), n_cross = 9u * nl;
+    const uint32_t n_s = 3u * (nl + 1u), n_self = 9u * (nl + 1u), n_cross = 9u * nl, n_nn0 = nl;
     const uint32_t iM = n_s, iAbs = n_s + 1u, iOne = n_s + 2u;
     if (k < n_s) {
         x = k;
@@ -125,8 +126,11 @@ __device__ __forceinline__ void record_pair(uint32_t nl, uint32_t k, uint32_t &x
         const uint32_t kk = k - n_s - n_self, l = kk / 9u + 1u, r = kk % 9u;
         x = 3u * l + r % 3u;
         y = 3u * (l - 1u) + r / 3u;
+    } else if (k < n_s + n_self + n_cross + n_nn0) {
+        x = 3u * (k - n_s - n_self - n_cross + 1u);  // NN of level l
+        y = 0u;                                       // NN of level 0
     } else {
-        const uint32_t m = k - n_s - n_self - n_cross;
+        const uint32_t m = k - n_s - n_self - n_cross - n_nn0;
         x = m == 0u ? iM : m == 1u ? iAbs : m == 2u ? iM : iOne;
         y = m == 2u ? iM : iOne;
     }

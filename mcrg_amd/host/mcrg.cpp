@@ -1,0 +1,474 @@
+// Implementation of the reference-named classes over the C ABI.
+#include "mcrg.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <iomanip>
+#include <mutex>
+#include <sstream>
+
+namespace mcrg_cpp {
+namespace {
+std::mutex g_mu;
+mcrg_ctx g_ctx = nullptr;
+int g_device = 0;
+uint64_t g_seed_base = 1;
+uint64_t g_seed_count = 0;
+
+uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+}  // namespace
+
+void check(mcrg_status st)
+{
+    if (st != MCRG_OK) throw Error(st, std::string("mcrg: ") + mcrg_last_error());
+}
+
+void set_device(int device)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_device = device;
+}
+
+mcrg_ctx context()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_ctx) check(mcrg_ctx_create(g_device, &g_ctx));
+    return g_ctx;
+}
+
+void set_seed(uint64_t base)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_seed_base = base;
+    g_seed_count = 0;
+}
+
+uint64_t next_seed()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return splitmix64(g_seed_base + g_seed_count++);
+}
+
+std::string rounded_str(double num, int precision)
+{
+    std::stringstream ss;
+    ss << std::setprecision(precision) << num;
+    return ss.str();
+}
+
+bool write_iter(int i)
+{
+    if (i < 10) return true;
+    if (i < 100) return i % 10 == 0;
+    if (i < 1000) return i % 100 == 0;
+    if (i < 10000) return i % 1000 == 0;
+    if (i < 100000) return i % 10000 == 0;
+    return i % 100000 == 0;
+}
+
+int split_samples(int rank, int n_processes, int n_samples)
+{
+    int loc = (int)std::ceil((double)n_samples / (double)n_processes);
+    if (rank == 0) loc = n_samples - (n_processes - 1) * loc;
+    return loc;
+}
+}  // namespace mcrg_cpp
+
+using mcrg_cpp::check;
+
+namespace {
+int levels_of(int N) { return std::max(0, (int)std::floor(std::log((double)N) / std::log(2.0)) - 1); }
+
+mcrg_chain new_chain(int N, double J, uint64_t seed, int n_levels, int spb, int bins)
+{
+    mcrg_chain_desc d;
+    d.L = N;
+    d.coupling = J;
+    d.seed = seed;
+    d.n_levels = n_levels;
+    d.layout = MCRG_LAYOUT_INT8;
+    d.sweeps_per_bin = spb;
+    d.max_bins = bins;
+    mcrg_chain ch = nullptr;
+    check(mcrg_chain_create(mcrg_cpp::context(), &d, &ch));
+    return ch;
+}
+}  // namespace
+
+// ------------------------------------------------------------------ Lattice
+Lattice::Lattice(int N) : N_(N), n_spins_(N * N), a_(1)
+{
+    seed_ = mcrg_cpp::next_seed();
+    chain_ = new_chain(N, 0.0, seed_, 0, 1, 1);
+    sync_spins();
+}
+
+Lattice::Lattice(int a, imat spins) : N_(spins.rows()), n_spins_(spins.rows() * spins.rows()), a_(a)
+{
+    spins_ = spins;
+    seed_ = mcrg_cpp::next_seed();
+    chain_ = new_chain(N_, 0.0, seed_, 0, 1, 1);
+    push_spins();
+}
+
+Lattice::~Lattice()
+{
+    if (chain_) mcrg_chain_destroy(chain_);
+}
+
+void Lattice::set_coupling(double J)
+{
+    if (J == J_) return;
+    check(mcrg_chain_set_coupling(chain_, J));
+    J_ = J;
+}
+
+void Lattice::sync_spins()
+{
+    std::vector<int8_t> s((size_t)n_spins_);
+    check(mcrg_chain_get_spins(chain_, s.data(), (int64_t)s.size()));
+    spins_.resize(N_, N_);
+    // device (x, y) at y*N + x == column-major (i, j) at j*N + i with i = x, j = y
+    for (int j = 0; j < N_; ++j)
+        for (int i = 0; i < N_; ++i) spins_(i, j) = s[(size_t)j * N_ + i];
+}
+
+void Lattice::push_spins()
+{
+    std::vector<int8_t> s((size_t)n_spins_);
+    for (int j = 0; j < N_; ++j)
+        for (int i = 0; i < N_; ++i) s[(size_t)j * N_ + i] = (int8_t)spins_(i, j);
+    check(mcrg_chain_set_spins(chain_, s.data(), (int64_t)s.size()));
+}
+
+void Lattice::write_spins(FILE *fptr)
+{
+    sync_spins();
+    std::fprintf(fptr, "\n");
+    for (int i = 0; i < N_; ++i) {
+        std::fprintf(fptr, "# ");
+        for (int j = 0; j < N_; ++j) std::fprintf(fptr, "%i, ", spins_(i, j));
+        std::fprintf(fptr, "\n");
+    }
+}
+
+void Lattice::display_spins()
+{
+    sync_spins();
+    for (int i = 0; i < N_; ++i) {
+        for (int j = 0; j < N_; ++j) std::printf(spins_(i, j) == 1 ? "\033[34m%2s\033[0m" : "\033[37m%2s\033[0m",
+                                                 spins_(i, j) == 1 ? "O" : "X");
+        std::printf("\n");
+    }
+    std::printf("(Lattice spacing a = %i)\n\n", a_);
+}
+
+int Lattice::choose_random_spin()
+{
+    // a uniform site index from the lattice's own counter-based sequence
+    uint64_t z = seed_ ^ (0xA0761D6478BD642Full * ++pick_);
+    z = (z ^ (z >> 32)) * 0xD6E8FEB86659FD93ull;
+    z ^= z >> 32;
+    return (int)(((z >> 11) * (double)n_spins_) / 9007199254740992.0);
+}
+
+double Lattice::calc_nearest_neighbor_interaction()
+{
+    int64_t ops[4];
+    check(mcrg_chain_operators(chain_, ops));
+    return 2.0 * (double)ops[0];
+}
+
+vec2D Lattice::calc_interactions()
+{
+    int64_t ops[4];
+    check(mcrg_chain_operators(chain_, ops));
+    vec2D S;
+    S(0) = 2.0 * (double)ops[0];
+    S(1) = 2.0 * (double)ops[1];
+    return S;
+}
+
+imat Lattice::nearest_neighbors(int i, int j)
+{
+    imat nn(4, 2);
+    const int N = N_;
+    nn(0, 0) = ((i + 1) % N + N) % N;
+    nn(0, 1) = j;
+    nn(1, 0) = ((i - 1) % N + N) % N;
+    nn(1, 1) = j;
+    nn(2, 0) = i;
+    nn(2, 1) = ((j + 1) % N + N) % N;
+    nn(3, 0) = i;
+    nn(3, 1) = ((j - 1) % N + N) % N;
+    return nn;
+}
+
+imat Lattice::next_nearest_neighbors(int i, int j)
+{
+    imat nnn(4, 2);
+    const int N = N_;
+    const int ip = ((i + 1) % N + N) % N, im = ((i - 1) % N + N) % N;
+    const int jp = ((j + 1) % N + N) % N, jm = ((j - 1) % N + N) % N;
+    nnn(0, 0) = ip;
+    nnn(0, 1) = jp;
+    nnn(1, 0) = im;
+    nnn(1, 1) = jp;
+    nnn(2, 0) = ip;
+    nnn(2, 1) = jm;
+    nnn(3, 0) = im;
+    nnn(3, 1) = jm;
+    return nnn;
+}
+
+// ------------------------------------------------------------------ IsingModel
+IsingModel::IsingModel(double K) : K_(K)
+{
+    if (!(K <= 0.0)) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "IsingModel: K must be <= 0 (ferromagnetic)");
+}
+
+void IsingModel::bind(Lattice &lat) { lat.set_coupling(-K_); }
+
+void IsingModel::sample_new_configuration(std::shared_ptr<Lattice> pLattice)
+{
+    bind(*pLattice);
+    check(mcrg_chain_sweep(pLattice->chain(), 1, 0));
+}
+
+void IsingModel::equilibrate(std::shared_ptr<Lattice> pLattice, int n_samples_eq, bool write)
+{
+    bind(*pLattice);
+    if (!write) {
+        check(mcrg_chain_sweep(pLattice->chain(), n_samples_eq, 0));
+        check(mcrg_chain_sync(pLattice->chain()));
+        return;
+    }
+    const std::string name = "equilibrate_N_" + std::to_string(pLattice->N_) + "_K_" +
+                             mcrg_cpp::rounded_str(K_, 7) + ".txt";
+    FILE *f = std::fopen(name.c_str(), "w");
+    if (!f) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "cannot open " + name);
+    std::fprintf(f, "# Nearest neighbor coupling K = %lf\n", K_);
+    std::fprintf(f, "# Temperature T = %lf\n", -1 / K_);
+    std::fprintf(f, "# Number of lattice sites = %i\n", pLattice->N_ * pLattice->N_);
+    std::fprintf(f, "# Lattice spacing = %i\n", pLattice->a_);
+    std::fprintf(f, "# Using %i parallel processes\n", 1);
+    std::fprintf(f, "# %s, %s, %s, %s, %s, %s, %s\n", "Iteration", "Avg E/spin", "Stddev E/spin", "Heat Capacity",
+                 "Avg |M|/spin", "Stddev |M|/spin", "Susceptibility");
+    for (int n = 1; n <= n_samples_eq; ++n) {
+        check(mcrg_chain_sweep(pLattice->chain(), 1, 0));
+        if (!mcrg_cpp::write_iter(n)) continue;
+        // one chain: the average over ranks of src/ising.cpp:50-63 is the value itself
+        const double E = calc_energy(pLattice), M = std::fabs(calc_magnetization(pLattice));
+        const double E_sigma = E * E - E * E, M_sigma = M * M - M * M;
+        std::fprintf(f, "%i, %10.7e, %10.7e, %10.7e, %10.7e, %10.7e, %10.7e\n", n, E, std::sqrt(E_sigma),
+                     E_sigma * K_ * K_, M, std::sqrt(M_sigma), -M_sigma * K_);
+    }
+    pLattice->write_spins(f);
+    std::fclose(f);
+}
+
+double IsingModel::calc_energy(std::shared_ptr<Lattice> pLattice)
+{
+    // K * (sum over sites of s times its 4 neighbours) / N^2, src/ising.cpp:158-173
+    int64_t ops[4];
+    check(mcrg_chain_operators(pLattice->chain(), ops));
+    return K_ * 2.0 * (double)ops[0] / ((double)pLattice->N_ * pLattice->N_);
+}
+
+double IsingModel::calc_magnetization(std::shared_ptr<Lattice> pLattice)
+{
+    int64_t ops[4];
+    check(mcrg_chain_operators(pLattice->chain(), ops));
+    return (double)ops[3] / ((double)pLattice->N_ * pLattice->N_);
+}
+
+// ------------------------------------------------------------------ MCRG
+MonteCarloRenormalizationGroup::MonteCarloRenormalizationGroup(int b) : b_(b)
+{
+    if (b != 2) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "only b = 2 majority-rule blocking is supported");
+    mcrg_cpp::context();
+    std::printf("\n=============================================\n");
+    std::printf("==========       MONTE CARLO       ==========\n");
+    std::printf("==========  RENORMALIZATION GROUP  ==========\n");
+    std::printf("=============================================\n\n");
+    std::printf("* Using %i parallel process(es)\n", n_processes_);
+    std::printf("* Scaling factor b = %i\n", b_);
+}
+
+void MonteCarloRenormalizationGroup::set_operators(int n_ops)
+{
+    if (n_ops < 1 || n_ops > 3) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "n_ops must be 1, 2 or 3");
+    n_ops_ = n_ops;
+}
+void MonteCarloRenormalizationGroup::set_replicas(int replicas)
+{
+    if (replicas < 1) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "replicas must be >= 1");
+    n_processes_ = replicas;
+}
+void MonteCarloRenormalizationGroup::set_bins(int bins)
+{
+    if (bins < 1) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "bins must be >= 1");
+    bins_ = bins;
+}
+void MonteCarloRenormalizationGroup::set_levels(int levels) { levels_ = levels; }
+void MonteCarloRenormalizationGroup::set_output_dir(const std::string &dir) { dir_ = dir; }
+void MonteCarloRenormalizationGroup::set_quiet(bool quiet) { quiet_ = quiet; }
+
+std::vector<int64_t> MonteCarloRenormalizationGroup::run_chains(int N, double K, int n_levels, int n_samples_eq,
+                                                                int n_samples, int &n_bins)
+{
+    const int R = n_processes_;
+    const int rs = mcrg_record_size(n_levels);
+    std::vector<int> n_loc((size_t)R);
+    int max_loc = 0;
+    for (int r = 0; r < R; ++r) {
+        n_loc[(size_t)r] = std::max(0, mcrg_cpp::split_samples(r, R, n_samples));
+        max_loc = std::max(max_loc, n_loc[(size_t)r]);
+    }
+    const int spb = std::max(1, (max_loc + bins_ - 1) / bins_);
+    n_bins = std::max(1, (max_loc + spb - 1) / spb);
+    std::vector<int64_t> total((size_t)n_bins * rs, 0);
+    const double J = -K;
+    if (N >= 8 && N <= 512) {
+        // replicas advance together on the LDS-resident schedule
+        std::vector<mcrg_job_desc> jobs((size_t)R);
+        for (int r = 0; r < R; ++r) {
+            mcrg_job_desc &j = jobs[(size_t)r];
+            j.L = N;
+            j.coupling = J;
+            j.seed = mcrg_cpp::next_seed();
+            j.n_therm = n_samples_eq;
+            j.n_meas = n_loc[(size_t)r];
+            j.n_levels = n_levels;
+            j.sweeps_per_bin = spb;
+        }
+        mcrg_batch b = nullptr;
+        check(mcrg_batch_create(mcrg_cpp::context(), jobs.data(), R, nullptr, &b));
+        int32_t left = 1;
+        while (left > 0) check(mcrg_batch_run(b, 1 << 30, &left));
+        for (int r = 0; r < R; ++r) {
+            const int nb = std::max(1, (n_loc[(size_t)r] + spb - 1) / spb);
+            std::vector<int64_t> rec((size_t)nb * rs);
+            check(mcrg_batch_get_records(b, r, rec.data(), (int64_t)rec.size()));
+            for (size_t k = 0; k < rec.size(); ++k) total[k] += rec[k];
+        }
+        mcrg_batch_destroy(b);
+    } else {
+        for (int r = 0; r < R; ++r) {
+            mcrg_chain ch = new_chain(N, J, mcrg_cpp::next_seed(), n_levels, spb, n_bins);
+            check(mcrg_chain_sweep(ch, n_samples_eq, 0));
+            check(mcrg_chain_sweep(ch, n_loc[(size_t)r], 1));
+            std::vector<int64_t> rec((size_t)n_bins * rs);
+            check(mcrg_chain_get_records(ch, rec.data(), (int64_t)rec.size()));
+            mcrg_chain_destroy(ch);
+            for (size_t k = 0; k < rec.size(); ++k) total[k] += rec[k];
+        }
+    }
+    return total;
+}
+
+void MonteCarloRenormalizationGroup::calc_critical_exponent(int n_samples_eq, int n_samples, int N, double K)
+{
+    if (!(K <= 0.0)) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "K must be <= 0 (ferromagnetic)");
+    const int n_t = levels_ >= 0 ? levels_ : levels_of(N);
+    if (n_t < 1) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "need at least one blocking level (N >= 4)");
+    if (!quiet_) {
+        std::printf("* Calculating critical exponent at K = %lf\n\n", K);
+        std::printf("Sampling %i configurations...\n", n_samples);
+    }
+    const std::string name = dir_ + "/critical_exponent_N_" + std::to_string(N) + "_K_" +
+                             mcrg_cpp::rounded_str(K, 7) + ".txt";
+    fptr_ = std::fopen(name.c_str(), "w");
+    if (!fptr_) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "cannot open " + name);
+    std::fprintf(fptr_, "# Number of parallel processes = %i\n", n_processes_);
+    std::fprintf(fptr_, "# Number of equilibration samples = %i\n", n_samples_eq);
+    std::fprintf(fptr_, "# Number of samples = %i\n", n_samples);
+    std::fprintf(fptr_, "# %23s  %25s  %25s\n", "Blocking Level n", "Largest Eigenvalue", "Critical Exponent nu");
+
+    int n_bins = 0;
+    const std::vector<int64_t> rec = run_chains(N, K, n_t, n_samples_eq, n_samples, n_bins);
+    results_.assign((size_t)n_t, mcrg_rg_result{});
+    std::string errs;
+    double nu = 0.0;
+    for (int n = 0; n < n_t; ++n) {
+        mcrg_rg_result r;
+        check(mcrg_rg_exponent(rec.data(), n_bins, n_t, n, n_ops_, &r));
+        results_[(size_t)n] = r;
+        nu = r.nu;
+        if (!quiet_) std::printf("n = %i: lambda = %lf, nu = %lf\n", n, r.lambda, r.nu);
+        std::fprintf(fptr_, "%25i, %25.10lf, %25.10lf\n", n, r.lambda, r.nu);
+    }
+    std::fclose(fptr_);
+    fptr_ = nullptr;
+    // jackknife errors beside the reference-format file
+    FILE *fe = std::fopen((name.substr(0, name.size() - 4) + ".err.txt").c_str(), "w");
+    if (fe) {
+        std::fprintf(fe, "# delete-one-bin jackknife over %i bins, %i operators\n", n_bins, n_ops_);
+        std::fprintf(fe, "# %23s  %25s  %25s  %25s\n", "Blocking Level n", "nu", "nu error", "y_t error");
+        for (int n = 0; n < n_t; ++n)
+            std::fprintf(fe, "%25i, %25.10lf, %25.10lf, %25.10lf\n", n, results_[(size_t)n].nu,
+                         results_[(size_t)n].nu_err, results_[(size_t)n].yt_err);
+        std::fclose(fe);
+    }
+    if (!quiet_) std::printf("\n* Critical exponent: nu = %lf\n", nu);
+}
+
+double MonteCarloRenormalizationGroup::approx_critical_point(int n_samples_eq, int n_samples, int L, double K)
+{
+    const int n_t = levels_of(L);
+    if (n_t < 1 || L / b_ < 4) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "L must be >= 8 to locate K_c");
+    int nb_l = 0, nb_s = 0;
+    const int saved_bins = bins_;
+    bins_ = 1;
+    const std::vector<int64_t> rl = run_chains(L, K, n_t, n_samples_eq, n_samples, nb_l);
+    const std::vector<int64_t> rs = run_chains(L / b_, K, n_t - 1, n_samples_eq, n_samples, nb_s);
+    bins_ = saved_bins;
+    last_kc_.assign((size_t)n_t, 0.0);
+    double Kc = K;
+    for (int n = 0; n < n_t; ++n) {
+        double dK = 0.0;
+        check(mcrg_critical_point_step(rl.data(), n_t, rs.data(), n_t - 1, n, &dK));
+        Kc = K + dK;
+        last_kc_[(size_t)n] = Kc;
+        if (!quiet_) std::printf("n = %i: Kc = %lf\n", n, Kc);
+        if (fptr_) {
+            std::fprintf(fptr_, "%25i, %25i, %25.10lf, %25.10lf\n", iter_, n, K, Kc);
+            std::fflush(fptr_);
+        }
+    }
+    return Kc;
+}
+
+double MonteCarloRenormalizationGroup::locate_critical_point(int n_iterations, int n_samples_eq, int n_samples,
+                                                              int L, double K0)
+{
+    if (!(K0 <= 0.0)) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "K0 must be <= 0 (ferromagnetic)");
+    if (!quiet_) std::printf("* Locating critical point starting from K0 = %lf\n", K0);
+    const std::string name = dir_ + "/critical_point_L_" + std::to_string(L) + "_K_" +
+                             mcrg_cpp::rounded_str(K0, 7) + ".txt";
+    fptr_ = std::fopen(name.c_str(), "w");
+    if (!fptr_) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "cannot open " + name);
+    std::fprintf(fptr_, "# Number of parallel processes = %i\n", n_processes_);
+    std::fprintf(fptr_, "# Number of equilibration samples = %i\n", n_samples_eq);
+    std::fprintf(fptr_, "# Number of samples = %i\n", n_samples);
+    std::fprintf(fptr_, "# %23s  %25s  %25s  %25s\n", "Iteration", "Blocking Level n", "Starting K", "Approximate Kc");
+    double K = K0;
+    for (iter_ = 1; iter_ <= n_iterations; ++iter_) {
+        if (!quiet_) std::printf("\nIteration %i:\n", iter_);
+        K = approx_critical_point(n_samples_eq, n_samples, L, K);
+    }
+    if (!quiet_) {
+        std::printf("\n* Critical point: Kc = %lf\n", K);
+        std::printf("* Critical temperature: Tc = %lf\n", -1.0 / K);
+    }
+    std::fclose(fptr_);
+    fptr_ = nullptr;
+    return K;
+}

@@ -1,0 +1,173 @@
+// C++ interface of the MCRG hot path with the reference's class and method
+// names (kim-jane/MCRG src/lattice.hpp, src/ising.hpp, src/mcrg.hpp), running on
+// the MI355X through the C ABI of libmcrg_amd.so (include/mcrg_abi.h).
+//
+// Differences from the reference, by design:
+//  * One sample (sample_new_configuration) is one red/black checkerboard
+//    Metropolis sweep (the reference flips one Wolff cluster).
+//  * Seeds are reproducible: every new chain takes the next value of a
+//    process-wide Philox seed sequence (mcrg_cpp::set_seed) instead of
+//    std::random_device.
+//  * No MPI.  The reference's ranks become replicas on the one device
+//    (MonteCarloRenormalizationGroup::set_replicas); samples are split over them
+//    with split_samples' rule and every replica equilibrates on its own.
+//  * imat / vec2D are small value types of this file instead of Eigen types.
+//  * calc_magnetization divides in floating point (the reference's integer
+//    division returns 0 or +-1, src/ising.cpp:178).
+//  * block_spin_transformation is fused into the device measurement; no blocked
+//    Lattice objects are built per sample.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mcrg_abi.h"
+
+// Column-major int matrix (the role of Eigen's imat, src/definitions.hpp:16).
+class imat {
+  public:
+    imat() = default;
+    imat(int rows, int cols) { resize(rows, cols); }
+    void resize(int rows, int cols)
+    {
+        r_ = rows;
+        c_ = cols;
+        v_.assign((size_t)rows * (size_t)cols, 0);
+    }
+    int rows() const { return r_; }
+    int cols() const { return c_; }
+    int &operator()(int i, int j) { return v_[(size_t)j * (size_t)r_ + (size_t)i]; }
+    int operator()(int i, int j) const { return v_[(size_t)j * (size_t)r_ + (size_t)i]; }
+    long long sum() const
+    {
+        long long s = 0;
+        for (int x : v_) s += x;
+        return s;
+    }
+    int *data() { return v_.data(); }
+    const int *data() const { return v_.data(); }
+
+  private:
+    int r_ = 0, c_ = 0;
+    std::vector<int> v_;
+};
+
+// Two-component vector (the role of Eigen's vec2D).
+struct vec2D {
+    double v[2] = {0.0, 0.0};
+    double &operator()(int i) { return v[i]; }
+    double operator()(int i) const { return v[i]; }
+};
+
+namespace mcrg_cpp {
+// Thrown for any non-OK status of the C ABI (message from mcrg_last_error()).
+class Error : public std::runtime_error {
+  public:
+    Error(int status, const std::string &what) : std::runtime_error(what), status(status) {}
+    int status;
+};
+void check(mcrg_status st);
+// Device used by every object of this process (default 0), set before the first object.
+void set_device(int device);
+mcrg_ctx context();
+// Start of the seed sequence (default 1); chain k of the process gets seed
+// splitmix64(base + k).
+void set_seed(uint64_t base);
+uint64_t next_seed();
+std::string rounded_str(double num, int precision);  // get_rounded_str, src/definitions.cpp:70-77
+bool write_iter(int i);                                // src/definitions.cpp:46-68
+int split_samples(int rank, int n_processes, int n_samples);  // src/definitions.cpp:79-87
+}  // namespace mcrg_cpp
+
+class Lattice {
+  public:
+    explicit Lattice(int N);     // random +-1 start (src/lattice.cpp:3-15)
+    Lattice(int a, imat spins);  // given spins, lattice spacing a (src/lattice.cpp:18-30)
+    ~Lattice();
+    Lattice(const Lattice &) = delete;
+    Lattice &operator=(const Lattice &) = delete;
+
+    int N_;
+    int n_spins_;
+    int a_;
+    imat spins_;  // host view of the device spins, refreshed by sync_spins()
+
+    void write_spins(FILE *fptr);
+    void display_spins();
+    int choose_random_spin();
+    double calc_nearest_neighbor_interaction();  // double-counted NN, as the reference
+    vec2D calc_interactions();                   // double-counted NN, NNN
+    imat nearest_neighbors(int i, int j);
+    imat next_nearest_neighbors(int i, int j);
+
+    // additions
+    void sync_spins();  // device -> spins_
+    void push_spins();  // spins_ -> device
+    mcrg_chain chain() const { return chain_; }
+    uint64_t seed() const { return seed_; }
+    void set_coupling(double J);  // J = -K; no-op when unchanged
+
+  private:
+    mcrg_chain chain_ = nullptr;
+    double J_ = 0.0;
+    uint64_t seed_ = 0;
+    uint64_t pick_ = 0;
+};
+
+class IsingModel {
+  public:
+    explicit IsingModel(double K);  // K < 0 ferromagnetic, as the reference
+    ~IsingModel() = default;
+
+    void equilibrate(std::shared_ptr<Lattice> pLattice, int n_samples_eq, bool write);
+    void sample_new_configuration(std::shared_ptr<Lattice> pLattice);
+    double calc_magnetization(std::shared_ptr<Lattice> pLattice);
+    double calc_energy(std::shared_ptr<Lattice> pLattice);
+
+  private:
+    double K_;
+    void bind(Lattice &lat);
+};
+
+class MonteCarloRenormalizationGroup {
+  public:
+    explicit MonteCarloRenormalizationGroup(int b);
+    ~MonteCarloRenormalizationGroup() = default;
+
+    int b_;
+
+    void calc_critical_exponent(int n_samples_eq, int n_samples, int N, double K);
+    double locate_critical_point(int n_iterations, int n_samples_eq, int n_samples, int L, double K0);
+
+    // additions
+    void set_operators(int n_ops);    // 2 (NN, NNN: the reference) or 3 (+ plaquette)
+    void set_replicas(int replicas);  // the reference's MPI ranks (default 1)
+    void set_bins(int bins);          // jackknife bins (default 50)
+    void set_levels(int levels);      // blockings; default floor(log2 N) - 1
+    void set_output_dir(const std::string &dir);
+    void set_quiet(bool quiet);
+    // exponents of the last calc_critical_exponent, one per level
+    const std::vector<mcrg_rg_result> &results() const { return results_; }
+    // per-level estimates of the last approx_critical_point
+    const std::vector<double> &last_kc() const { return last_kc_; }
+
+  private:
+    double approx_critical_point(int n_samples_eq, int n_samples, int L, double K);
+    // records of `replicas` chains of side N at coupling K (summed per bin)
+    std::vector<int64_t> run_chains(int N, double K, int n_levels, int n_samples_eq, int n_samples, int &n_bins);
+
+    int n_processes_ = 1;
+    int n_ops_ = 2;
+    int bins_ = 50;
+    int levels_ = -1;
+    int iter_ = 0;
+    bool quiet_ = false;
+    std::string dir_ = ".";
+    FILE *fptr_ = nullptr;
+    std::vector<mcrg_rg_result> results_;
+    std::vector<double> last_kc_;
+};

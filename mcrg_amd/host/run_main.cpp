@@ -1,0 +1,89 @@
+// `mcrg_run`: the reference's `run` driver (src/main.cpp:4-22) on the MI355X.
+// With no arguments it runs the reference's call sequence: b = 2, N = 128,
+// 1e4 equilibration and 1e6 measured samples, calc_critical_exponent at
+// K = -0.44 and at K_c = -ln(1 + sqrt 2)/2.  Flags change the sizes and add the
+// critical-point locator (src/mcrg.cpp:146-188).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mcrg.hpp"
+
+static void usage()
+{
+    std::printf(
+        "usage: mcrg_run [--N n] [--eq n] [--samples n] [--K k ...] [--replicas r] [--ops 2|3]\n"
+        "                [--bins b] [--levels l] [--seed s] [--device d] [--outdir dir] [--quiet]\n"
+        "                [--locate iterations --L l --K0 k]\n"
+        "K values are in the reference's sign convention (K < 0 ferromagnetic); 'Kc' means\n"
+        "-ln(1+sqrt(2))/2.\n");
+}
+
+int main(int argc, char **argv)
+{
+    int b = 2, N = 128, n_eq = 10000, n = 1000000, replicas = 1, ops = 2, bins = 50, levels = -1, device = 0;
+    int locate = 0, Lloc = 16;
+    double K0 = -0.44;
+    unsigned long long seed = 1;
+    bool quiet = false;
+    std::string outdir = ".";
+    const double Kc = -std::log(1.0 + std::sqrt(2.0)) / 2.0;
+    std::vector<double> Ks;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        auto next = [&](void) -> const char * {
+            if (i + 1 >= argc) {
+                usage();
+                std::exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "--N") N = std::atoi(next());
+        else if (a == "--eq") n_eq = std::atoi(next());
+        else if (a == "--samples") n = std::atoi(next());
+        else if (a == "--K") {
+            const std::string v = next();
+            Ks.push_back(v == "Kc" ? Kc : std::atof(v.c_str()));
+        } else if (a == "--replicas") replicas = std::atoi(next());
+        else if (a == "--ops") ops = std::atoi(next());
+        else if (a == "--bins") bins = std::atoi(next());
+        else if (a == "--levels") levels = std::atoi(next());
+        else if (a == "--seed") seed = std::strtoull(next(), nullptr, 10);
+        else if (a == "--device") device = std::atoi(next());
+        else if (a == "--outdir") outdir = next();
+        else if (a == "--quiet") quiet = true;
+        else if (a == "--locate") locate = std::atoi(next());
+        else if (a == "--L") Lloc = std::atoi(next());
+        else if (a == "--K0") {
+            const std::string v = next();
+            K0 = v == "Kc" ? Kc : std::atof(v.c_str());
+        } else {
+            usage();
+            return a == "--help" || a == "-h" ? 0 : 2;
+        }
+    }
+    if (Ks.empty()) Ks = {-0.44, Kc};
+    try {
+        mcrg_cpp::set_device(device);
+        mcrg_cpp::set_seed(seed);
+        MonteCarloRenormalizationGroup MCRG(b);
+        MCRG.set_replicas(replicas);
+        MCRG.set_operators(ops);
+        MCRG.set_bins(bins);
+        MCRG.set_levels(levels);
+        MCRG.set_output_dir(outdir);
+        MCRG.set_quiet(quiet);
+        if (locate > 0) {
+            MCRG.locate_critical_point(locate, n_eq, n, Lloc, K0);
+        } else {
+            for (double K : Ks) MCRG.calc_critical_exponent(n_eq, n, N, K);
+        }
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "mcrg_run: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

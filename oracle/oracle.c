@@ -253,6 +253,8 @@ void orc_measure(int L, const int8_t *s, int nl, uint64_t seed, uint32_t t, int6
                 if (l >= 1) rec[ORC_OFF_CROSS(nl, l, a, b)] += S[l][a] * S[l - 1][b];
             }
         }
+        /* locator input: blocked NN against level-0 NN (src/mcrg.cpp:262-263) */
+        if (l >= 1) rec[ORC_OFF_NN0(nl, l)] += S[l][0] * S[0][0];
     }
     int64_t M = orc_magnetization(L, s);
     rec[ORC_OFF_M(nl) + 0] += M;
@@ -433,6 +435,25 @@ int orc_jackknife_yt(const int64_t *records, int n_bins, int nl, int level, int 
     free(tot);
     free(loo);
     free(yi);
+    return 0;
+}
+
+/* One matching step of approx_critical_point (src/mcrg.cpp:284-299) at level n:
+ * the L lattice blocked n+1 times against the L/2 lattice blocked n times, NN
+ * operator, with the reference's double-counted values (2 x single count). */
+int orc_kc_step(const int64_t *rl, int nll, const int64_t *rs, int nls, int n, double *dK)
+{
+    const double NL = (double)rl[ORC_OFF_N(nll)], NS = (double)rs[ORC_OFF_N(nls)];
+    const double SL = 2.0 * rl[ORC_OFF_S(nll, 0, 0)] / NL;
+    const double SS = 2.0 * rs[ORC_OFF_S(nls, 0, 0)] / NS;
+    const double SLb = 2.0 * rl[ORC_OFF_S(nll, n + 1, 0)] / NL;
+    const double SSb = 2.0 * rs[ORC_OFF_S(nls, n, 0)] / NS;
+    const double SLb_SL = 4.0 * rl[ORC_OFF_NN0(nll, n + 1)] / NL;
+    const double SSb_SS = 4.0 * (n == 0 ? rs[ORC_OFF_SELF(nls, 0, 0, 0)] : rs[ORC_OFF_NN0(nls, n)]) / NS;
+    const double dSL_dK = SLb_SL - SLb * SL;
+    const double dSS_dK = SSb_SS - SSb * SS;
+    if (dSL_dK - dSS_dK == 0.0) return -1;
+    *dK = (SLb - SSb) / (dSL_dK - dSS_dK);
     return 0;
 }
 

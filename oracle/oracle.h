@@ -14,6 +14,7 @@
  *   src/mcrg.cpp:314-348   blocking    orc_block
  *   src/mcrg.cpp:72-98     accumulate  orc_measure
  *   src/mcrg.cpp:100-137   T, lambda   orc_rg_matrix, orc_leading_eigenvalue, orc_yt
+ *   src/mcrg.cpp:284-299   K_c step    orc_kc_step
  *   src/definitions.cpp:9-19 flatten   record layout (column-major 3x3 blocks)
  *   src/definitions.cpp:79-87 split    orc_split_samples
  *
@@ -51,7 +52,9 @@
  *            NN  = sum_(x,y) s(x,y) [s(x+1,y) + s(x,y+1)]
  *            NNN = sum_(x,y) [s(x,y) s(x+1,y+1) + s(x+1,y) s(x,y+1)]
  *            P   = sum_(x,y) s(x,y) s(x+1,y) s(x,y+1) s(x+1,y+1)
- *  Record    per bin, int64, for levels l = 0..nl (see ORC_* offsets below).
+ *  Record    per bin, int64, for levels l = 0..nl (see ORC_* offsets below):
+ *            sums of S_l, of S_l S_l^T, of S_l S_(l-1)^T (l >= 1), of
+ *            NN_l NN_0 (l >= 1), of M, |M|, M^2, and the sample count.
  */
 #ifndef MCRG_ORACLE_H
 #define MCRG_ORACLE_H
@@ -68,7 +71,8 @@ enum { ORC_TAG_SWEEP = 1, ORC_TAG_TIE = 2, ORC_TAG_INIT = 3 };
 #define ORC_OFF_S(nl, l, a)        ((l) * 3 + (a))
 #define ORC_OFF_SELF(nl, l, a, b)  (3 * ((nl) + 1) + (l) * 9 + (b) * 3 + (a))
 #define ORC_OFF_CROSS(nl, l, a, b) (12 * ((nl) + 1) + ((l) - 1) * 9 + (b) * 3 + (a))
-#define ORC_OFF_M(nl)              (12 * ((nl) + 1) + 9 * (nl))
+#define ORC_OFF_NN0(nl, l)         (12 * ((nl) + 1) + 9 * (nl) + (l) - 1)
+#define ORC_OFF_M(nl)              (12 * ((nl) + 1) + 10 * (nl))
 #define ORC_OFF_N(nl)              (ORC_OFF_M(nl) + 3)
 #define ORC_RECORD_SIZE(nl)        (ORC_OFF_N(nl) + 1)
 
@@ -94,6 +98,8 @@ double orc_leading_eigenvalue(int n, const double *T);
 double orc_yt(const int64_t *rec, int nl, int level, int n_ops);
 int    orc_jackknife_yt(const int64_t *records, int n_bins, int nl, int level, int n_ops,
                         double *mean, double *err);
+int    orc_kc_step(const int64_t *rec_large, int nl_large, const int64_t *rec_small, int nl_small,
+                   int level, double *dK);
 int    orc_split_samples(int rank, int n_processes, int n_samples);
 
 #ifdef __cplusplus

@@ -50,6 +50,7 @@ def lib():
             "orc_jackknife_yt": (c_int, [i64p, c_int, c_int, c_int, c_int,
                                          ctypes.POINTER(c_d), ctypes.POINTER(c_d)]),
             "orc_split_samples": (c_int, [c_int, c_int, c_int]),
+            "orc_kc_step": (c_int, [i64p, c_int, i64p, c_int, c_int, ctypes.POINTER(c_d)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -146,6 +147,15 @@ def jackknife_yt(records, nl, level, n_ops):
     return m.value, e.value
 
 
+def kc_step(rec_large, nl_large, rec_small, nl_small, level):
+    d = ctypes.c_double()
+    rc = lib().orc_kc_step(np.ascontiguousarray(rec_large, dtype=np.int64), nl_large,
+                           np.ascontiguousarray(rec_small, dtype=np.int64), nl_small, level, ctypes.byref(d))
+    if rc != 0:
+        raise ValueError("undefined step")
+    return d.value
+
+
 def split_samples(rank, n_processes, n_samples):
     return lib().orc_split_samples(rank, n_processes, n_samples)
 
@@ -163,8 +173,12 @@ def off_cross(nl, l, a, b):
     return 12 * (nl + 1) + (l - 1) * 9 + b * 3 + a
 
 
+def off_nn0(nl, l):
+    return 12 * (nl + 1) + 9 * nl + l - 1
+
+
 def off_M(nl):
-    return 12 * (nl + 1) + 9 * nl
+    return 12 * (nl + 1) + 10 * nl
 
 
 def off_N(nl):

@@ -61,4 +61,4 @@ def test_thresholds_reject_antiferromagnetic():
 @pytest.mark.parametrize("nl", [0, 1, 5, 10])
 def test_record_size_matches_oracle(orc, nl):
     import mcrg_amd
-    assert mcrg_amd.record_size(nl) == orc.record_size(nl) == 21 * nl + 16
+    assert mcrg_amd.record_size(nl) == orc.record_size(nl) == 22 * nl + 16

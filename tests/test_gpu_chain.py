@@ -34,14 +34,14 @@ def _say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("L", [4, 8, 16, 32, 64, 128, 256, 2048])
+@pytest.mark.parametrize("L", [2, 4, 8, 16, 32, 64, 128, 256, 2048])
 def test_init_matches_oracle(amd, ctx, orc, L):
     _say("init", L)
     with amd.Chain(ctx, L, KC, seed=12345 + L) as ch:
         assert np.array_equal(ch.spins(), orc.init_spins(L, 12345 + L))
 
 
-@pytest.mark.parametrize("L,J", [(4, KC), (8, KC), (8, 0.0), (16, 0.3), (32, 0.489652),
+@pytest.mark.parametrize("L,J", [(2, KC), (4, KC), (8, KC), (8, 0.0), (16, 0.3), (32, 0.489652),
                                  (64, KC), (128, 0.4320459), (256, 2.5), (512, KC)])
 def test_sweeps_bit_exact(amd, ctx, orc, L, J):
     seed = 0xDEADBEEF00000000 + L
@@ -87,7 +87,7 @@ def test_measurement_records_bit_exact(amd, ctx, orc, L, nl, J):
         assert ch.state() == (n_therm + n_meas, n_meas)
 
 
-@pytest.mark.parametrize("L", [4, 16, 64, 512])
+@pytest.mark.parametrize("L", [2, 4, 16, 64, 512])
 def test_operators_match_oracle(amd, ctx, orc, L):
     seed = 5 + L
     with amd.Chain(ctx, L, KC, seed=seed) as ch:
@@ -145,7 +145,7 @@ def test_set_coupling_matches_oracle(amd, ctx, orc):
 
 
 @pytest.mark.parametrize("kw,msg", [
-    (dict(L=12), "power of two"), (dict(L=2), "power of two"), (dict(coupling=-0.44), "coupling"),
+    (dict(L=12), "power of two"), (dict(L=1), "power of two"), (dict(coupling=-0.44), "coupling"),
     (dict(n_levels=7, L=64), "n_levels"), (dict(sweeps_per_bin=0), "sweeps_per_bin"),
     (dict(L=4096, n_levels=2), "2048"),
 ])

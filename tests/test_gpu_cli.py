@@ -1,0 +1,114 @@
+"""The reference-named C++ classes (mcrg_amd/host) and the `mcrg_run` driver on
+the GPU: bit-exact against the oracle through the classes, output files in the
+reference's formats (src/mcrg.cpp:31-39,135 and :157-166,303), and the same
+exponents as the batch entry points with the same seeds."""
+import json
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "mcrg_amd", "mcrg_run")
+PROBE = os.path.join(ROOT, "mcrg_amd", "_build", "class_probe")
+KC_REF = -math.log(1.0 + math.sqrt(2.0)) / 2.0
+
+
+def _run(args, cwd):
+    print("run", args, flush=True)
+    out = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+def test_classes_bit_exact_vs_oracle(orc, tmp_path):
+    from mcrg_amd.workloads import splitmix64
+    N, K = 16, -0.4406868
+    got = json.loads(_run([PROBE, str(N), repr(K)], tmp_path))
+    seed = splitmix64(12345)
+    assert got["seed"] == seed
+    s = orc.init_spins(N, seed)
+    for t in range(37 + 5):
+        orc.sweep(N, s, seed, -K, t)
+    assert got["spins"] == [int(v) for v in s]
+    ops = orc.ops(N, s)
+    assert got["NN_ref"] == 2 * ops[0] and got["NNN_ref"] == 2 * ops[1] and got["NN_only"] == 2 * ops[0]
+    # calc_interactions is the reference's double-counted sum, src/lattice.cpp:102-120
+    ref = orc.ops_refcount(N, s)
+    assert got["NN_ref"] == ref[0] and got["NNN_ref"] == ref[1]
+    assert got["E"] == pytest.approx(K * 2 * ops[0] / N**2, rel=1e-12)
+    assert got["M"] == pytest.approx(s.astype(float).sum() / N**2, rel=1e-12)
+    assert got["nn"] == [[1, N - 1], [N - 1, N - 1], [0, 0], [0, N - 2]]
+    assert got["nnn"] == [[1, 0], [N - 1, 0], [1, N - 2], [N - 1, N - 2]]
+
+
+def _read_rows(path):
+    rows = [l for l in open(path).read().splitlines() if not l.startswith("#")]
+    return [[float(x) for x in r.split(",")] for r in rows]
+
+
+def test_critical_exponent_file_and_batch_parity(tmp_path):
+    import mcrg_amd
+    from mcrg_amd.workloads import splitmix64
+    N, n_eq, n, seed = 16, 200, 4000, 5
+    out = _run([RUN, "--N", str(N), "--eq", str(n_eq), "--samples", str(n), "--K", "Kc", "--seed", str(seed),
+                "--outdir", str(tmp_path)], tmp_path)
+    assert "MONTE CARLO" in out and "RENORMALIZATION GROUP" in out
+    assert "* Critical exponent: nu =" in out
+    name = tmp_path / "critical_exponent_N_16_K_-0.4406868.txt"
+    text = open(name).read().splitlines()
+    assert text[0] == "# Number of parallel processes = 1"
+    assert text[1] == f"# Number of equilibration samples = {n_eq}"
+    assert text[2] == f"# Number of samples = {n}"
+    assert text[3] == "# %23s  %25s  %25s" % ("Blocking Level n", "Largest Eigenvalue", "Critical Exponent nu")
+    rows = _read_rows(name)
+    assert [int(r[0]) for r in rows] == [0, 1, 2]  # floor(log2 16) - 1 levels
+    for line in text[4:]:
+        assert len(line) == 25 + 2 + 25 + 2 + 25
+    # the same chain through the batch entry points: same seed, levels and bins
+    job = dict(L=N, coupling=-KC_REF, seed=splitmix64(seed), n_therm=n_eq, n_meas=n, n_levels=3,
+               sweeps_per_bin=(n + 49) // 50)
+    with mcrg_amd.Context(0) as ctx, mcrg_amd.Batch(ctx, [job]) as b:
+        b.run_all()
+        rec = b.records(0)
+    for lvl, row in enumerate(rows):
+        r = mcrg_amd.rg_exponent(rec, 3, lvl, 2)
+        assert row[1] == pytest.approx(r["lambda"], abs=1e-9)
+        assert row[2] == pytest.approx(r["nu"], abs=1e-9)
+    err = _read_rows(tmp_path / "critical_exponent_N_16_K_-0.4406868.err.txt")
+    assert len(err) == 3 and all(e[2] > 0 for e in err)
+
+
+def test_replicas_and_hbm_schedule(tmp_path):
+    # N = 4 runs on the HBM schedule (below the LDS-resident minimum), 3 replicas
+    _run([RUN, "--N", "4", "--eq", "50", "--samples", "300", "--K", "-0.3", "--replicas", "3",
+          "--outdir", str(tmp_path), "--quiet"], tmp_path)
+    text = open(tmp_path / "critical_exponent_N_4_K_-0.3.txt").read()
+    assert "# Number of parallel processes = 3" in text
+    assert len(_read_rows(tmp_path / "critical_exponent_N_4_K_-0.3.txt")) == 1
+
+
+def test_locate_critical_point_file(tmp_path):
+    out = _run([RUN, "--locate", "2", "--L", "16", "--K0", "-0.44", "--eq", "200", "--samples", "3000",
+                "--outdir", str(tmp_path)], tmp_path)
+    assert "* Critical point: Kc =" in out
+    path = tmp_path / "critical_point_L_16_K_-0.44.txt"
+    text = open(path).read().splitlines()
+    assert text[3] == "# %23s  %25s  %25s  %25s" % ("Iteration", "Blocking Level n", "Starting K",
+                                                    "Approximate Kc")
+    rows = _read_rows(path)
+    assert [(int(r[0]), int(r[1])) for r in rows] == [(1, 0), (1, 1), (1, 2), (2, 0), (2, 1), (2, 2)]
+    assert rows[0][2] == pytest.approx(-0.44)
+    assert rows[3][2] == pytest.approx(rows[2][3])  # iteration 2 starts from iteration 1's deepest level
+    for r in rows:
+        assert -0.5 < r[3] < -0.38
+
+
+def test_cli_rejects_bad_coupling(tmp_path):
+    out = subprocess.run([RUN, "--N", "16", "--K", "0.3", "--eq", "1", "--samples", "10", "--outdir",
+                          str(tmp_path)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode != 0 and "ferromagnetic" in out.stderr

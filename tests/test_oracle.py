@@ -217,7 +217,8 @@ def test_rg_matrix_converges_to_exact_4x4(orc):
 # ------------------------------------------------------------------ records
 def test_record_layout_against_per_sample_sums(orc):
     """Recompute every record entry with Python integers from per-sample operator
-    values (S(n) S(n-1)^T flattened column-major, as src/definitions.cpp:9-19)."""
+    values (S(n) S(n-1)^T flattened column-major, as src/definitions.cpp:9-19;
+    NN(n) NN(0) as src/mcrg.cpp:262-263)."""
     L, nl, J, seed = 8, 2, KC, 5
     s0 = orc.init_spins(L, seed)
     _, rec = orc.run_chain(L, J, seed, n_therm=3, n_meas=7, nl=nl, sweeps_per_bin=100, s=s0)
@@ -239,6 +240,8 @@ def test_record_layout_against_per_sample_sums(orc):
                     want[orc.off_self(nl, l, a, b)] += S[l][a] * S[l][b]
                     if l >= 1:
                         want[orc.off_cross(nl, l, a, b)] += S[l][a] * S[l - 1][b]
+            if l >= 1:
+                want[orc.off_nn0(nl, l)] += S[l][0] * S[0][0]
         M = int(s.astype(np.int64).sum())
         want[orc.off_M(nl)] += M
         want[orc.off_M(nl) + 1] += abs(M)

@@ -1,0 +1,103 @@
+"""The product's FP64 solve (host code in libmcrg_amd.so) against the oracle's
+independent implementation, numpy, and closed forms.  No device work."""
+import math
+
+import numpy as np
+import pytest
+
+KC = 0.5 * math.log(1.0 + math.sqrt(2.0))
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import mcrg_amd
+    return mcrg_amd
+
+
+@pytest.fixture(scope="module")
+def chains(orc):
+    """Oracle records for a few (L, J, levels)."""
+    out = {}
+    for L, J, nl in ((8, KC, 2), (16, 0.42, 3), (16, KC, 3), (8, 0.3, 1)):
+        _, rec = orc.run_chain(L, J, seed=L * 1000 + nl, n_therm=200, n_meas=4000, nl=nl, sweeps_per_bin=500)
+        out[(L, J, nl)] = rec
+    return out
+
+
+def test_rg_matrix_matches_oracle(amd, orc, chains):
+    for (L, J, nl), rec in chains.items():
+        tot = rec.sum(axis=0)
+        for level in range(nl):
+            for n_ops in (1, 2, 3):
+                a = amd.rg_matrix(tot, nl, level, n_ops)
+                b = orc.rg_matrix(tot, nl, level, n_ops)
+                assert np.allclose(a, b, rtol=1e-9, atol=1e-12), (L, J, nl, level, n_ops)
+
+
+def test_leading_eigenvalue_against_numpy(amd):
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 3):
+        for _ in range(500):
+            T = rng.normal(size=(n, n)) * rng.choice([0.01, 1.0, 30.0])
+            want = max(np.linalg.eigvals(T).real)
+            assert abs(amd.leading_eigenvalue(T) - want) < 1e-9 * max(1.0, abs(want))
+
+
+def test_leading_eigenvalue_closed_forms(amd):
+    assert amd.leading_eigenvalue(np.diag([2.0, 3.0, -7.0])) == pytest.approx(3.0)
+    rot = np.array([[1.0, -2.0, 0.0], [2.0, 1.0, 0.0], [0.0, 0.0, 0.5]])  # 1 +- 2i, 0.5
+    assert amd.leading_eigenvalue(rot) == pytest.approx(1.0)
+    rot2 = np.array([[0.2, -2.0, 0.0], [2.0, 0.2, 0.0], [0.0, 0.0, 0.9]])  # 0.2 +- 2i, 0.9
+    assert amd.leading_eigenvalue(rot2) == pytest.approx(0.9)
+    jordan = np.array([[2.0, 1.0, 0.0], [0.0, 2.0, 1.0], [0.0, 0.0, 2.0]])
+    assert amd.leading_eigenvalue(jordan) == pytest.approx(2.0, rel=1e-5)
+
+
+def test_exponent_and_jackknife_match_oracle(amd, orc, chains):
+    for (L, J, nl), rec in chains.items():
+        for level in range(nl):
+            for n_ops in (2, 3):
+                got = amd.rg_exponent(rec, nl, level, n_ops)
+                m, e = orc.jackknife_yt(rec, nl, level, n_ops)
+                assert got["yt"] == pytest.approx(m, rel=1e-9)
+                assert got["yt_err"] == pytest.approx(e, rel=1e-6, abs=1e-12)
+                assert got["nu"] == pytest.approx(1.0 / m, rel=1e-9)
+                assert got["lambda"] == pytest.approx(2.0 ** m, rel=1e-9)
+                assert got["n_jackknife"] == rec.shape[0]
+
+
+def test_critical_point_step_matches_oracle(amd, orc):
+    L, J = 16, 0.44
+    _, rl = orc.run_chain(L, J, seed=1, n_therm=300, n_meas=3000, nl=3, sweeps_per_bin=3000)
+    _, rs = orc.run_chain(L // 2, J, seed=2, n_therm=300, n_meas=3000, nl=2, sweeps_per_bin=3000)
+    for level in range(3):
+        got = amd.critical_point_step(rl[0], 3, rs[0], 2, level)
+        want = orc.kc_step(rl[0], 3, rs[0], 2, level)
+        assert got == pytest.approx(want, rel=1e-9)
+
+
+def test_critical_point_step_is_zero_for_identical_statistics(amd, orc):
+    # Matching a lattice against itself: blocked L statistics equal the
+    # unblocked "small" ones exactly when both records are built to agree.
+    nl = 2
+    rec = np.zeros(orc.record_size(nl), dtype=np.int64)
+    n = 100
+    rec[orc.off_N(nl)] = n
+    rec[orc.off_S(nl, 0, 0)] = 50 * n
+    rec[orc.off_S(nl, 1, 0)] = 30 * n
+    rec[orc.off_nn0(nl, 1)] = 1600 * n
+    rec[orc.off_self(nl, 0, 0, 0)] = 2600 * n
+    small = np.zeros(orc.record_size(nl - 1), dtype=np.int64)
+    small[orc.off_N(nl - 1)] = n
+    small[orc.off_S(nl - 1, 0, 0)] = 30 * n
+    small[orc.off_self(nl - 1, 0, 0, 0)] = 1100 * n
+    # <S_L^(1)> = <S_S^(0)>: the numerator vanishes
+    assert amd.critical_point_step(rec, nl, small, nl - 1, 0) == 0.0
+
+
+def test_solve_rejects_bad_arguments(amd):
+    rec = np.zeros(amd.record_size(2), dtype=np.int64)
+    with pytest.raises(amd.MCRGError):
+        amd.rg_matrix(rec, 2, 0, 2)  # no samples
+    with pytest.raises(amd.MCRGError):
+        amd.rg_matrix(rec, 2, 2, 2)  # level out of range
