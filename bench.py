@@ -19,6 +19,13 @@ Measured beside it (same JSON line):
                HIP-event-timed average duration (events on its own stream)
   cpu_baseline the oracle (scalar C) on the host's cores over a sample of the
                manifest (threads over independent replicas)
+  serving      open-loop Poisson job streams (mcrg_amd.workloads.serving_stream:
+               L = 2^k, k in 4..9, |K| ~ U[0.42, 0.47], 1e3 thermalisation plus
+               U[1e3, 2e4] measured sweeps, blocking to L = 4, seed = job id)
+               submitted to the job server at each --serving-rates rate for
+               --serving-duration seconds; reports p50/p99 latency (submit to
+               records on the host, y_t computed by the client) and aggregate
+               spin updates / ns from the first arrival to the last completion
 """
 import argparse
 import concurrent.futures as cf
@@ -101,6 +108,77 @@ def run_plain_2048(mcrg_amd, ctx, steps):
     return L * L * steps / (dt * 1e9)
 
 
+def run_serving(mcrg_amd, workloads, ctx, rate, duration, drain_limit, quantum, threads):
+    stream = workloads.serving_stream(rate, duration)
+    srv = mcrg_amd.Server(ctx, launch_quantum=quantum, threads=threads)
+    try:
+        return _serve(mcrg_amd, srv, stream, rate, duration, drain_limit, quantum)
+    finally:
+        srv.close()  # stops the scheduler thread
+
+
+def _serve(mcrg_amd, srv, stream, rate, duration, drain_limit, quantum):
+    srv.start()
+    lat, yts, updates = [], [], 0.0
+    yt_fail = [0]
+    n_done = 0
+    next_log = 10.0
+
+    def collect():
+        nonlocal n_done, updates
+        for jid in srv.completed():
+            job = srv.jobs[jid]
+            rec = srv.records(jid)
+            nl = job["n_levels"]
+            try:  # y_t from the 2-operator RG matrix between L/2 and L/4
+                yts.append(mcrg_amd.rg_exponent(rec, nl, 1, 2)["yt"])
+            except mcrg_amd.MCRGError:
+                yt_fail[0] += 1
+            p = srv.poll(jid)
+            lat.append(p["finish_s"] - p["submit_s"])
+            updates += job["L"] ** 2 * (job["n_therm"] + job["n_meas"])
+            srv.release(jid)
+            del srv.jobs[jid]
+            n_done += 1
+
+    t0 = time.perf_counter()
+    late = 0.0
+    for ta, job in stream:
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        else:
+            late = max(late, now - ta)
+        srv.submit(job)
+        if time.perf_counter() - t0 > next_log:
+            collect()
+            log(f"bench: serving {rate}/s t={time.perf_counter() - t0:.0f}s submitted "
+                f"{srv.stats()['submitted']} done {n_done}")
+            next_log += 10.0
+    t_last_arrival = time.perf_counter() - t0
+    while n_done < len(stream) and time.perf_counter() - t0 < duration + drain_limit:
+        collect()
+        time.sleep(0.005)
+        if time.perf_counter() - t0 > next_log:
+            log(f"bench: serving {rate}/s draining t={time.perf_counter() - t0:.0f}s done {n_done}/{len(stream)}")
+            next_log += 10.0
+    collect()
+    t_end = time.perf_counter() - t0
+    st = srv.stats()
+    srv.stop()
+    lat_ms = sorted(1e3 * x for x in lat)
+
+    def pct(q):
+        return lat_ms[min(len(lat_ms) - 1, int(q * len(lat_ms)))] if lat_ms else None
+
+    return {"rate_per_s": rate, "duration_s": duration, "jobs": len(stream), "completed": n_done,
+            "p50_ms": pct(0.50), "p99_ms": pct(0.99), "max_ms": lat_ms[-1] if lat_ms else None,
+            "updates_per_ns": updates / (t_end * 1e9), "wall_s": t_end, "last_arrival_s": t_last_arrival,
+            "max_submit_lag_s": late, "launches": st["launches"], "busy_s": st["busy_s"],
+            "yt_median": sorted(yts)[len(yts) // 2] if yts else None, "yt_undefined": yt_fail[0],
+            "launch_quantum": quantum or (1 << 24)}
+
+
 def cpu_baseline(jobs, sweeps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
@@ -131,6 +209,12 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serving-rates", default="200",
+                    help="comma-separated arrival rates in jobs/s for the serving leg ('' = none)")
+    ap.add_argument("--serving-duration", type=float, default=10.0)
+    ap.add_argument("--serving-drain", type=float, default=120.0,
+                    help="seconds after the last arrival to wait for completions")
+    ap.add_argument("--serving-quantum", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the replicas are "
@@ -162,6 +246,13 @@ def main():
         log(f"bench: plain path on {len(sample)} replicas x 20 sweeps: {plain_mixed:.3f} updates/ns")
         plain_2048 = run_plain_2048(mcrg_amd, ctx, 500)
         log(f"bench: plain L=2048: {plain_2048:.1f} updates/ns")
+    serving = []
+    for rate in [float(r) for r in args.serving_rates.split(",") if r.strip()]:
+        res = run_serving(mcrg_amd, workloads, ctx, rate, args.serving_duration, args.serving_drain,
+                          args.serving_quantum, args.threads)
+        log(f"bench: serving {rate}/s: {res['completed']}/{res['jobs']} jobs, p50 {res['p50_ms']} ms, "
+            f"p99 {res['p99_ms']} ms, {res['updates_per_ns']:.1f} updates/ns")
+        serving.append(res)
     ctx.close()
 
     cpu = None
@@ -194,6 +285,7 @@ def main():
         "plain_mixed": plain_mixed,
         "plain_l2048": plain_2048,
         "batched_over_plain_mixed": (value / plain_mixed) if plain_mixed else None,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -179,6 +179,75 @@ mcrg_status mcrg_batch_get_records(mcrg_batch b, int32_t job, int64_t *host, int
 mcrg_status mcrg_batch_get_spins(mcrg_batch b, int32_t job, int8_t *host, int64_t n);
 mcrg_status mcrg_batch_get_state(mcrg_batch b, int32_t job, uint32_t *sweep, uint32_t *n_measured);
 
+/* ------------------------------------------------------------------ serving */
+/* Continuous batching of MCRG jobs (mcrg_job_desc) on one device.  Jobs are
+ * submitted at any time from any thread; a scheduler admits queued jobs in
+ * arrival order while their spins and records fit the device-memory budget and
+ * their slot share of LDS fits the LDS budget, advances every admitted job by
+ * one time slice per launch of the LDS-resident schedule (jobs of different L
+ * and coupling share each launch; a slot of G lattices of side L runs
+ * min(sweeps_per_launch, max(1, launch_quantum / (G L^2))) sweeps, so every
+ * workgroup does about launch_quantum site updates), and retires finished jobs
+ * between launches: their records are copied to host memory and their device
+ * memory is reused by the next admissions.  A job's results are bit-identical
+ * to the same job run alone (its own seed and counters).  No reference
+ * counterpart: the reference runs one calc_critical_exponent per process
+ * (src/main.cpp:15-17). */
+typedef struct mcrg_server_s *mcrg_server;
+
+typedef struct {
+    int64_t hbm_budget_bytes;  /* device memory for resident jobs; 0 = 4 GiB          */
+    int64_t lds_budget_bytes;  /* summed LDS share of resident jobs; 0 = two full
+                                  waves of workgroups (2 x CUs x LDS per CU)          */
+    int64_t launch_quantum;    /* site updates per workgroup per launch; 0 = 2^24     */
+    int32_t max_jobs;          /* jobs resident at once; 0 = 65536                    */
+    int32_t threads;           /* workgroup size; 0 = 256                             */
+    int32_t sweeps_per_launch; /* most sweeps of one job per launch; 0 = 65536        */
+    int32_t reserved;          /* 0                                                   */
+} mcrg_server_opts;
+
+enum { MCRG_JOB_QUEUED = 0, MCRG_JOB_RUNNING = 1, MCRG_JOB_DONE = 2, MCRG_JOB_CANCELLED = 3 };
+
+typedef struct {
+    int32_t state;         /* MCRG_JOB_*                                        */
+    int32_t n_bins;        /* record bins of the job                            */
+    double submit_s;       /* times in seconds since the server was created     */
+    double start_s;        /* admission (0 while queued)                        */
+    double finish_s;       /* records available (0 until done)                  */
+} mcrg_job_status;
+
+typedef struct {
+    int64_t submitted, admitted, completed, cancelled, queued, running;
+    int64_t launches;
+    double site_updates;   /* completed sweeps x L^2 over all jobs              */
+    double busy_s;         /* time with at least one job running                */
+    int64_t hbm_in_use;    /* bytes of the budget held by resident jobs         */
+    int64_t lds_in_use;    /* LDS shares of the resident jobs                   */
+} mcrg_server_stats;
+
+mcrg_status mcrg_server_create(mcrg_ctx ctx, const mcrg_server_opts *opts, mcrg_server *out);
+/* Stops the scheduler if running and frees everything. */
+mcrg_status mcrg_server_destroy(mcrg_server s);
+/* Queues a job; *job_id identifies it from now on.  MCRG_ERR_INVALID_ARG for a
+ * malformed job, MCRG_ERR_OUT_OF_BUDGET for one that can never fit. */
+mcrg_status mcrg_server_submit(mcrg_server s, const mcrg_job_desc *job, uint64_t *job_id);
+mcrg_status mcrg_server_poll(mcrg_server s, uint64_t job_id, mcrg_job_status *out);
+/* Records of a finished job, n = n_bins * mcrg_record_size(n_levels);
+ * MCRG_ERR_NOT_READY until it is done. */
+mcrg_status mcrg_server_records(mcrg_server s, uint64_t job_id, int64_t *host, int64_t n);
+/* Drops a finished job's host results, or cancels a queued or running one. */
+mcrg_status mcrg_server_release(mcrg_server s, uint64_t job_id);
+/* Ids of jobs finished since the last call, in completion order (at most
+ * max_ids; the rest stay for the next call).  *n receives the count. */
+mcrg_status mcrg_server_completed(mcrg_server s, uint64_t *ids, int32_t max_ids, int32_t *n);
+/* Runs the scheduler on a thread of its own until mcrg_server_stop. */
+mcrg_status mcrg_server_start(mcrg_server s);
+mcrg_status mcrg_server_stop(mcrg_server s);
+/* One scheduling iteration on the calling thread (when not started):
+ * retire, admit, launch one chunk, wait for it.  *active = jobs resident. */
+mcrg_status mcrg_server_step(mcrg_server s, int32_t *active);
+mcrg_status mcrg_server_stats_get(mcrg_server s, mcrg_server_stats *out);
+
 /* ------------------------------------------------------------------ FP64 solve */
 /* Host arithmetic on records; no device work.  The connected correlators are
  * formed exactly from the int64 sums in 128-bit integers, then in FP64.

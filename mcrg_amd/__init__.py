@@ -27,6 +27,10 @@ ERR_NOT_FOUND = -5
 ERR_CAPACITY = -6
 LAYOUT_INT8 = 0
 LAYOUT_PACKED = 1
+JOB_QUEUED = 0
+JOB_RUNNING = 1
+JOB_DONE = 2
+JOB_CANCELLED = 3
 
 
 class MCRGError(RuntimeError):
@@ -55,6 +59,24 @@ class BatchOpts(ctypes.Structure):
 class RGResult(ctypes.Structure):
     _fields_ = [("lambda_", ctypes.c_double), ("yt", ctypes.c_double), ("nu", ctypes.c_double),
                 ("yt_err", ctypes.c_double), ("nu_err", ctypes.c_double), ("n_jackknife", ctypes.c_int32)]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("lds_budget_bytes", ctypes.c_int64),
+                ("launch_quantum", ctypes.c_int64), ("max_jobs", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("sweeps_per_launch", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class JobStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("n_bins", ctypes.c_int32), ("submit_s", ctypes.c_double),
+                ("start_s", ctypes.c_double), ("finish_s", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("admitted", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("cancelled", ctypes.c_int64), ("queued", ctypes.c_int64), ("running", ctypes.c_int64),
+                ("launches", ctypes.c_int64), ("site_updates", ctypes.c_double), ("busy_s", ctypes.c_double),
+                ("hbm_in_use", ctypes.c_int64), ("lds_in_use", ctypes.c_int64)]
 
 
 class DeviceInfo(ctypes.Structure):
@@ -94,6 +116,17 @@ _SIGS = {
     "mcrg_batch_get_records": (_c.c_int32, [_p, _c.c_int32, _p, _c.c_int64]),
     "mcrg_batch_get_spins": (_c.c_int32, [_p, _c.c_int32, _p, _c.c_int64]),
     "mcrg_batch_get_state": (_c.c_int32, [_p, _c.c_int32, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint32)]),
+    "mcrg_server_create": (_c.c_int32, [_p, _c.POINTER(ServerOpts), _c.POINTER(_p)]),
+    "mcrg_server_destroy": (_c.c_int32, [_p]),
+    "mcrg_server_submit": (_c.c_int32, [_p, _c.POINTER(JobDesc), _c.POINTER(_c.c_uint64)]),
+    "mcrg_server_poll": (_c.c_int32, [_p, _c.c_uint64, _c.POINTER(JobStatus)]),
+    "mcrg_server_records": (_c.c_int32, [_p, _c.c_uint64, _p, _c.c_int64]),
+    "mcrg_server_release": (_c.c_int32, [_p, _c.c_uint64]),
+    "mcrg_server_completed": (_c.c_int32, [_p, _c.POINTER(_c.c_uint64), _c.c_int32, _c.POINTER(_c.c_int32)]),
+    "mcrg_server_start": (_c.c_int32, [_p]),
+    "mcrg_server_stop": (_c.c_int32, [_p]),
+    "mcrg_server_step": (_c.c_int32, [_p, _c.POINTER(_c.c_int32)]),
+    "mcrg_server_stats_get": (_c.c_int32, [_p, _c.POINTER(ServerStats)]),
     "mcrg_rg_matrix": (_c.c_int32, [_p, _c.c_int32, _c.c_int32, _c.c_int32, _c.POINTER(_c.c_double)]),
     "mcrg_leading_eigenvalue": (_c.c_double, [_c.c_int32, _c.POINTER(_c.c_double)]),
     "mcrg_rg_exponent": (_c.c_int32, [_p, _c.c_int32, _c.c_int32, _c.c_int32, _c.c_int32, _c.POINTER(RGResult)]),
@@ -322,6 +355,87 @@ class Batch:
     def close(self):
         if self.handle:
             _lib.mcrg_batch_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Server:
+    """Continuous-batching job server (mcrg_server): jobs are admitted while the
+    device budget has room, advance one chunk per launch of the LDS-resident
+    schedule and retire to host memory when done.  Drive it with step() from
+    this thread, or start() a scheduler thread of its own."""
+
+    def __init__(self, ctx, hbm_budget_bytes=0, lds_budget_bytes=0, launch_quantum=0, max_jobs=0, threads=0,
+                 sweeps_per_launch=0):
+        opts = ServerOpts(hbm_budget_bytes, lds_budget_bytes, launch_quantum, max_jobs, threads, sweeps_per_launch,
+                          0)
+        h = ctypes.c_void_p()
+        check(_lib.mcrg_server_create(ctx.handle, ctypes.byref(opts), ctypes.byref(h)))
+        self.handle = h
+        self.jobs = {}
+
+    def submit(self, job):
+        d = JobDesc(job["L"], float(job["coupling"]), job["seed"], job.get("n_therm", 0), job.get("n_meas", 0),
+                    job.get("n_levels", 0), job.get("sweeps_per_bin", 1))
+        jid = ctypes.c_uint64()
+        check(_lib.mcrg_server_submit(self.handle, ctypes.byref(d), ctypes.byref(jid)))
+        self.jobs[jid.value] = dict(job)
+        return jid.value
+
+    def poll(self, jid):
+        st = JobStatus()
+        check(_lib.mcrg_server_poll(self.handle, jid, ctypes.byref(st)))
+        return dict(state=st.state, n_bins=st.n_bins, submit_s=st.submit_s, start_s=st.start_s,
+                    finish_s=st.finish_s)
+
+    def records(self, jid):
+        job = self.jobs[jid]
+        rs = record_size(job.get("n_levels", 0))
+        nb = n_bins(job.get("n_meas", 0), job.get("sweeps_per_bin", 1))
+        a = np.zeros(nb * rs, dtype=np.int64)
+        check(_lib.mcrg_server_records(self.handle, jid, _ptr(a), a.size))
+        return a.reshape(nb, rs)
+
+    def release(self, jid):
+        check(_lib.mcrg_server_release(self.handle, jid))
+
+    def completed(self, max_ids=4096):
+        """Ids of jobs finished since the last call."""
+        buf = (ctypes.c_uint64 * max_ids)()
+        n = ctypes.c_int32()
+        check(_lib.mcrg_server_completed(self.handle, buf, max_ids, ctypes.byref(n)))
+        return list(buf[:n.value])
+
+    def step(self):
+        active = ctypes.c_int32()
+        check(_lib.mcrg_server_step(self.handle, ctypes.byref(active)))
+        return active.value
+
+    def start(self):
+        check(_lib.mcrg_server_start(self.handle))
+
+    def stop(self):
+        check(_lib.mcrg_server_stop(self.handle))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self):
+        st = ServerStats()
+        check(_lib.mcrg_server_stats_get(self.handle, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            _lib.mcrg_server_destroy(self.handle)
             self.handle = None
 
     def __enter__(self):

@@ -140,6 +140,19 @@ int ResidentEngine::group_size(int L, int nl) const
     return std::max(G, 1);
 }
 
+uint32_t ResidentEngine::lds_share(int L, int nl) const
+{
+    const int G = group_size(L, nl);
+    return (lds_bytes(L, G, nl) + (uint32_t)G - 1u) / (uint32_t)G;
+}
+
+uint32_t ResidentEngine::slot_sweeps(int L, int G, uint32_t chunk) const
+{
+    if (opt_.quantum == 0) return chunk;
+    const uint64_t n = opt_.quantum / ((uint64_t)G * (uint64_t)L * (uint64_t)L);
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n, chunk));
+}
+
 int ResidentEngine::add(const JobSpec &j)
 {
     int id;
@@ -281,7 +294,8 @@ void ResidentEngine::rebuild_slots(uint32_t chunk)
             // positions: a power of two (so that teams tile the workgroup), empty ones are -1
             int gp = 1;
             while (gp < g) gp *= 2;
-            Slot s{L, gp, nl, (int32_t)slot_lats_.size()};
+            const uint32_t ns = slot_sweeps(L, gp, chunk);
+            Slot s{L, gp, nl, (int32_t)slot_lats_.size(), ns};
             uint64_t work = 0;
             for (int k = 0; k < gp; ++k) {
                 if (k >= g) {
@@ -291,7 +305,7 @@ void ResidentEngine::rebuild_slots(uint32_t chunk)
                 const int id = ids[a + (size_t)k];
                 slot_lats_.push_back(id);
                 const Job &j = jobs_[(size_t)id];
-                work += (uint64_t)L * (uint64_t)L * std::min<uint64_t>(chunk, j.therm_left + j.meas_left);
+                work += (uint64_t)L * (uint64_t)L * std::min<uint64_t>(ns, j.therm_left + j.meas_left);
             }
             pend.push_back({s, work});
             max_lds_ = std::max(max_lds_, lds_bytes(L, gp, nl));
@@ -356,8 +370,8 @@ hipError_t ResidentEngine::tick(uint32_t n, uint32_t &advanced, KernelTimer *tm)
     }
     if (slots_.empty()) return hipSuccess;
     const int ev = tm ? tm->begin(kKindResident, stream_) : -1;
-    e = launch_resident(opt_.threads, d_slots_, (uint32_t)slots_.size(), d_slot_lats_, d_lats_, d_queue_, chunk,
-                        max_lds_, (uint32_t)grid_, stream_);
+    e = launch_resident(opt_.threads, d_slots_, (uint32_t)slots_.size(), d_slot_lats_, d_lats_, d_queue_, max_lds_,
+                        (uint32_t)grid_, stream_);
     if (tm) tm->end(ev, stream_);
     if (e != hipSuccess) {
         failed_ = "launch_resident";
@@ -365,18 +379,22 @@ hipError_t ResidentEngine::tick(uint32_t n, uint32_t &advanced, KernelTimer *tm)
     }
     // mirror the device counters
     bool any_done = false;
-    for (Job &j : jobs_) {
-        if (!j.live) continue;
-        const uint32_t total = j.therm_left + j.meas_left;
-        if (total == 0) continue;
-        const uint32_t swept = std::min(total, chunk);
-        const uint32_t therm = std::min(swept, j.therm_left);
-        const uint32_t meas = swept - therm;
-        j.t += swept;
-        j.therm_left -= therm;
-        j.meas_left -= meas;
-        j.n_measured += meas;
-        if (j.therm_left + j.meas_left == 0) any_done = true;
+    for (const Slot &sl : slots_) {
+        for (int k = 0; k < sl.G; ++k) {
+            const int id = slot_lats_[(size_t)sl.first + (size_t)k];
+            if (id < 0) continue;
+            Job &j = jobs_[(size_t)id];
+            const uint32_t total = j.therm_left + j.meas_left;
+            if (total == 0) continue;
+            const uint32_t swept = std::min(total, sl.n_sweeps);
+            const uint32_t therm = std::min(swept, j.therm_left);
+            const uint32_t meas = swept - therm;
+            j.t += swept;
+            j.therm_left -= therm;
+            j.meas_left -= meas;
+            j.n_measured += meas;
+            if (j.therm_left + j.meas_left == 0) any_done = true;
+        }
     }
     if (any_done) slots_dirty_ = true;
     advanced = chunk;

@@ -44,8 +44,12 @@ class Arena {
 
 struct EngineOptions {
     int threads = 256;           // workgroup size of the resident kernel
-    int sweeps_per_launch = 250;  // chunk length
+    int sweeps_per_launch = 250;  // chunk length (the most sweeps any lattice runs per launch)
     int max_group = 256;          // upper bound on lattices per slot
+    // Time slicing: when nonzero, a slot of G positions of side L runs
+    // min(chunk, max(1, quantum / (G L^2))) sweeps per launch, so that every
+    // workgroup does about `quantum` site updates; 0 = every slot runs the chunk.
+    uint64_t quantum = 0;
 };
 
 class ResidentEngine {
@@ -86,6 +90,8 @@ class ResidentEngine {
     const char *failed_call() const { return failed_; }
     int group_size(int L, int nl) const;
     uint32_t lds_bytes(int L, int G, int nl) const;
+    // LDS bytes one lattice of this shape takes in its slot (slot LDS / G).
+    uint32_t lds_share(int L, int nl) const;
 
   private:
     struct Job {
@@ -97,6 +103,7 @@ class ResidentEngine {
         uint32_t t = 0, n_measured = 0, therm_left = 0, meas_left = 0;
     };
     void rebuild_slots(uint32_t chunk);
+    uint32_t slot_sweeps(int L, int G, uint32_t chunk) const;
     hipError_t upload_slots();
 
     int device_ = 0;

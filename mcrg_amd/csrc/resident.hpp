@@ -34,6 +34,7 @@ struct PackedLattice {
 // -1 marks an empty position.
 struct Slot {
     int32_t L, G, n_levels, first;
+    uint32_t n_sweeps;  // sweeps this launch advances each lattice of the slot (at most)
 };
 
 __host__ __device__ inline uint32_t cb_words(uint32_t L) { return L * L / 64u; }
@@ -80,7 +81,7 @@ __host__ __device__ inline SlotLayout slot_layout(uint32_t L, uint32_t G, uint32
 
 hipError_t launch_packed_init(PackedLattice *lats, const int32_t *idx, int n, hipStream_t s);
 hipError_t launch_resident(int threads, const Slot *slots, uint32_t n_slots, const int32_t *slot_lats,
-                           PackedLattice *lats, uint32_t *queue, uint32_t n_chunk, uint32_t lds_bytes,
+                           PackedLattice *lats, uint32_t *queue, uint32_t lds_bytes,
                            uint32_t grid, hipStream_t s);
 int resident_max_active_groups(int threads, uint32_t lds_bytes);
 

@@ -1,5 +1,5 @@
 // Body of the LDS-resident schedule: one workgroup advances the G lattices of
-// one slot by up to n_chunk sweeps, measuring the sweeps that fall in each
+// one slot by up to the slot's n_sweeps sweeps, measuring the sweeps that fall in each
 // lattice's measurement phase.
 //
 // Each lattice is owned by a team of T = nt / G consecutive threads (lane u of
@@ -299,9 +299,10 @@ __device__ __forceinline__ void pyramid(const uint32_t *lg, const uint32_t *cg, 
 #endif
 
 __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
-                         PackedLattice *__restrict__ lats, uint32_t n_chunk, uint32_t *lds, Team tm)
+                         PackedLattice *__restrict__ lats, uint32_t *lds, Team tm)
 {
     const uint32_t L = (uint32_t)sl.L, G = (uint32_t)sl.G, nl = (uint32_t)sl.n_levels;
+    const uint32_t n_chunk = sl.n_sweeps;
     const uint32_t T = tm.T;
     const SlotLayout lay = slot_layout(L, G, nl, T);
     const uint32_t W = lay.W, rs = record_size(nl), nw = lay.team_waves;

@@ -36,8 +36,7 @@ __global__ __launch_bounds__(256) void k_packed_init(PackedLattice *__restrict__
 template <int NT>
 __global__ __launch_bounds__(NT) void k_resident(const Slot *__restrict__ slots, uint32_t n_slots,
                                                  const int32_t *__restrict__ slot_lats,
-                                                 PackedLattice *__restrict__ lats, uint32_t *queue,
-                                                 uint32_t n_chunk)
+                                                 PackedLattice *__restrict__ lats, uint32_t *queue)
 {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_idx;
@@ -56,7 +55,7 @@ __global__ __launch_bounds__(NT) void k_resident(const Slot *__restrict__ slots,
         tm.us = tm.T;
         tm.lane = threadIdx.x;
         tm.block = tm.T > 64u;
-        run_slot(sl, slot_lats, lats, n_chunk, lds, tm);
+        run_slot(sl, slot_lats, lats, lds, tm);
     }
 }
 
@@ -73,8 +72,8 @@ hipError_t launch_packed_init(PackedLattice *lats, const int32_t *idx, int n, hi
 
 template <int NT>
 static hipError_t launch_nt(const Slot *slots, uint32_t n_slots, const int32_t *slot_lats,
-                            PackedLattice *lats, uint32_t *queue, uint32_t n_chunk, uint32_t lds_bytes,
-                            uint32_t grid, hipStream_t s)
+                            PackedLattice *lats, uint32_t *queue, uint32_t lds_bytes, uint32_t grid,
+                            hipStream_t s)
 {
     // above 64 KiB of dynamic LDS the kernel has to be opted in
     static uint32_t opted_in = 65536u;
@@ -86,19 +85,19 @@ static hipError_t launch_nt(const Slot *slots, uint32_t n_slots, const int32_t *
     }
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, queue);
     hipLaunchKernelGGL(k_resident<NT>, dim3(grid), dim3(NT), lds_bytes, s, slots, n_slots, slot_lats,
-                       lats, queue, n_chunk);
+                       lats, queue);
     return hipGetLastError();
 }
 
 hipError_t launch_resident(int threads, const Slot *slots, uint32_t n_slots, const int32_t *slot_lats,
-                           PackedLattice *lats, uint32_t *queue, uint32_t n_chunk, uint32_t lds_bytes,
-                           uint32_t grid, hipStream_t s)
+                           PackedLattice *lats, uint32_t *queue, uint32_t lds_bytes, uint32_t grid,
+                           hipStream_t s)
 {
     switch (threads) {
-    case 64: return launch_nt<64>(slots, n_slots, slot_lats, lats, queue, n_chunk, lds_bytes, grid, s);
-    case 128: return launch_nt<128>(slots, n_slots, slot_lats, lats, queue, n_chunk, lds_bytes, grid, s);
-    case 256: return launch_nt<256>(slots, n_slots, slot_lats, lats, queue, n_chunk, lds_bytes, grid, s);
-    case 512: return launch_nt<512>(slots, n_slots, slot_lats, lats, queue, n_chunk, lds_bytes, grid, s);
+    case 64: return launch_nt<64>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
+    case 128: return launch_nt<128>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
+    case 256: return launch_nt<256>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
+    case 512: return launch_nt<512>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
