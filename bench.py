@@ -40,6 +40,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "spin-updates/ns (batched vs plain) at matched y_t error; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of k_resident<threads> for (replicas, sweeps per launch,
+# threads), from two rocprofv3 --pmc passes over `bench.py --only batched`:
+# FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 KiB per dispatch, the scale factors read
+# off the same counters on a 1 GiB read + 1 GiB write copy kernel
+# (--copy-calibration).  Other shapes report null.
+MEASURED_TRAFFIC = {(4096, 100, 256): (3178.7 * 2.0 + 5028.8) * 1024.0}
 
 
 def log(*a):
@@ -197,6 +203,20 @@ def cpu_baseline(jobs, sweeps, threads):
                       f"measured sweeps each, {threads} threads over independent replicas: {dt:.1f} s"}
 
 
+def copy_calibration(n_bytes):
+    """A plain elementwise kernel that reads and writes n_bytes each (torch.neg
+    on int32), five times: the reference point for the FETCH_SIZE / WRITE_SIZE
+    counters of a profiled run."""
+    import torch
+    x = torch.ones(n_bytes // 4, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    log(f"bench: copy calibration {n_bytes} bytes read + {n_bytes} bytes written per launch, 5 launches")
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,7 +235,13 @@ def main():
     ap.add_argument("--serving-drain", type=float, default=120.0,
                     help="seconds after the last arrival to wait for completions")
     ap.add_argument("--serving-quantum", type=int, default=0)
+    ap.add_argument("--only", choices=["batched", "plain_l2048"], default=None,
+                    help="run one leg alone and print its line to stderr (for profiler runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the counter-calibration copy kernel and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the replicas are "
               "independent chains and this build runs on a single device. Run with --gpus 1.",
@@ -227,6 +253,15 @@ def main():
 
     jobs = workloads.batch_manifest(args.replicas)
     ctx = mcrg_amd.Context(0)
+    if args.only == "batched":
+        value, dt, _ = run_batched(mcrg_amd, ctx, jobs, args.steps, args.warmup, args.sweeps_per_step, args.threads)
+        log(f"bench: batched {value:.1f} updates/ns ({dt:.3f} s timed)")
+        ctx.close()
+        return 0
+    if args.only == "plain_l2048":
+        log(f"bench: plain L=2048: {run_plain_2048(mcrg_amd, ctx, args.steps):.1f} updates/ns")
+        ctx.close()
+        return 0
     dev = ctx.device_info()
     log(f"bench: {dev['name']} {dev['compute_units']} CUs; {len(jobs)} replicas, "
         f"{args.warmup}+{args.steps} steps x {args.sweeps_per_step} sweeps")
@@ -280,7 +315,9 @@ def main():
                    "replicas": len(jobs), "sweeps_per_step": args.sweeps_per_step, "threads": args.threads,
                    "sites": sites, "layout": "packed", "schedule": "resident", "device": dev["name"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_resident<256>",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((len(jobs), args.sweeps_per_step, args.threads)),
+                     "kernel": f"k_resident<{args.threads}>",
                      "avg_ms": avg_ms, "bytes_per_launch": bpl},
         "plain_mixed": plain_mixed,
         "plain_l2048": plain_2048,

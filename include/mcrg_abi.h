@@ -8,7 +8,7 @@
  * The reference (kim-jane/MCRG) has no FFI: its boundary is three C++ classes.
  * Each entry point below names the reference member it stands in for; the
  * C++ classes that keep the reference's names sit on top of this ABI
- * (mcrg_amd/csrc/host/).
+ * (mcrg_amd/host/).
  *
  * Model and stream conventions: coupling J = |K| > 0 is ferromagnetic (the
  * reference writes K = -J, src/ising.cpp:9); spins are +1/-1 int8 in row-major

@@ -281,3 +281,65 @@ def test_split_samples(orc):
     assert orc.split_samples(0, 8, 10) == -4  # the reference's quirk is kept
     assert orc.split_samples(3, 8, 10) == 2
     assert sum(orc.split_samples(r, 4, 1000003) for r in range(4)) == 1000003
+
+
+# ------------------------------------------------------------------ jackknife, critical-point step
+def _numpy_yt(rec, nl, level, n_ops):
+    """y_t from one record with numpy: connected correlators, T = A^-1 B,
+    log2 of the largest real eigenvalue (src/mcrg.cpp:110-131)."""
+    n = float(rec[orc_off_N(nl)])
+    S = lambda l: np.array([rec[l * 3 + a] for a in range(n_ops)], dtype=float) / n
+    self_ = np.array([[rec[3 * (nl + 1) + (level + 1) * 9 + b * 3 + a] for b in range(n_ops)]
+                      for a in range(n_ops)], dtype=float) / n
+    cross = np.array([[rec[12 * (nl + 1) + level * 9 + b * 3 + a] for b in range(n_ops)]
+                      for a in range(n_ops)], dtype=float) / n
+    A = self_ - np.outer(S(level + 1), S(level + 1))
+    B = cross - np.outer(S(level + 1), S(level))
+    lam = max(np.linalg.eigvals(np.linalg.solve(A, B)).real)
+    return math.log2(lam)
+
+
+def orc_off_N(nl):
+    return 12 * (nl + 1) + 10 * nl + 3
+
+
+def test_jackknife_against_numpy(orc):
+    nl, n_ops = 2, 2
+    _, rec = orc.run_chain(8, KC, seed=5, n_therm=200, n_meas=4000, nl=nl, sweeps_per_bin=500)
+    assert rec.shape[0] == 8
+    tot = rec.sum(axis=0)
+    for level in range(nl):
+        assert orc.yt(tot, nl, level, n_ops) == pytest.approx(_numpy_yt(tot, nl, level, n_ops), rel=1e-9)
+        loo = [_numpy_yt(tot - rec[b], nl, level, n_ops) for b in range(8)]
+        want_err = math.sqrt(7.0 / 8.0 * sum((y - np.mean(loo)) ** 2 for y in loo))
+        m, e = orc.jackknife_yt(rec, nl, level, n_ops)
+        assert m == pytest.approx(_numpy_yt(tot, nl, level, n_ops), rel=1e-9)
+        assert e == pytest.approx(want_err, rel=1e-6)
+    # identical bins: no spread
+    same = np.repeat(rec[:1], 5, axis=0)
+    m, e = orc.jackknife_yt(same, nl, 0, n_ops)
+    assert e == pytest.approx(0.0, abs=1e-12) and m == pytest.approx(orc.yt(rec[0], nl, 0, n_ops), rel=1e-12)
+
+
+def kc_known_answer_records(orc):
+    """Two samples per lattice, single-count NN values chosen by hand.
+    Large (levels 0..2): NN_0 = 10, 14 and NN_1 = 6, 4.  Small (levels 0..1):
+    NN_0 = 5, 7.  With the reference's double-counted S (src/mcrg.cpp:240-267):
+    <SL> = 24, <SLb_0> = 10, <SLb_0 SL> = (12*20 + 8*28)/2 = 232 -> dSL = -8;
+    <SS> = <SSb_0> = 12, <SSb_0 SS> = (100 + 196)/2 = 148 -> dSS = 4;
+    dK = (10 - 12) / (-8 - 4) = 1/6 (src/mcrg.cpp:292-295)."""
+    rl = np.zeros(orc.record_size(2), dtype=np.int64)
+    rl[orc.off_N(2)] = 2
+    rl[orc.off_S(2, 0, 0)] = 10 + 14
+    rl[orc.off_S(2, 1, 0)] = 6 + 4
+    rl[orc.off_nn0(2, 1)] = 6 * 10 + 4 * 14
+    rs = np.zeros(orc.record_size(1), dtype=np.int64)
+    rs[orc.off_N(1)] = 2
+    rs[orc.off_S(1, 0, 0)] = 5 + 7
+    rs[orc.off_self(1, 0, 0, 0)] = 25 + 49
+    return rl, rs
+
+
+def test_kc_step_known_answer(orc):
+    rl, rs = kc_known_answer_records(orc)
+    assert orc.kc_step(rl, 2, rs, 1, 0) == pytest.approx(1.0 / 6.0, rel=1e-12)

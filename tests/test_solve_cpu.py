@@ -101,3 +101,9 @@ def test_solve_rejects_bad_arguments(amd):
         amd.rg_matrix(rec, 2, 0, 2)  # no samples
     with pytest.raises(amd.MCRGError):
         amd.rg_matrix(rec, 2, 2, 2)  # level out of range
+
+
+def test_critical_point_step_known_answer(amd, orc):
+    from test_oracle import kc_known_answer_records
+    rl, rs = kc_known_answer_records(orc)
+    assert amd.critical_point_step(rl, 2, rs, 1, 0) == pytest.approx(1.0 / 6.0, rel=1e-12)
