@@ -163,6 +163,9 @@ typedef struct {
     int32_t threads;           /* workgroup size 64/128/256/512; 0 = 256       */
     int32_t sweeps_per_launch; /* sweeps per kernel launch; 0 = 250           */
     int32_t max_group;         /* upper bound on lattices per workgroup; 0 = 256 */
+    int32_t max_slot_lds;      /* LDS bytes a workgroup of small lattices may take
+                                  (lattices per workgroup halve until it fits);
+                                  0 = 32 KiB per 256 threads                    */
 } mcrg_batch_opts;
 
 mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n_jobs,

@@ -34,6 +34,8 @@ mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n
         if (opts->threads) eo.threads = opts->threads;
         if (opts->sweeps_per_launch) eo.sweeps_per_launch = opts->sweeps_per_launch;
         if (opts->max_group) eo.max_group = opts->max_group;
+        MCRG_REQUIRE(opts->max_slot_lds >= 0, "max_slot_lds must be >= 0");
+        eo.max_slot_lds = (uint32_t)opts->max_slot_lds;
     }
     MCRG_REQUIRE(eo.threads == 64 || eo.threads == 128 || eo.threads == 256 || eo.threads == 512,
                  "threads must be 64, 128, 256 or 512");

@@ -74,6 +74,14 @@ void Arena::free(size_t off, size_t bytes)
 ResidentEngine::~ResidentEngine()
 {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    for (hipStream_t a : aux_)
+        if (a) {
+            (void)hipStreamSynchronize(a);
+            (void)hipStreamDestroy(a);
+        }
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    for (hipEvent_t e : ev_join_)
+        if (e) (void)hipEventDestroy(e);
     arena_.release();
     if (d_lats_) (void)hipFree(d_lats_);
     if (d_slots_) (void)hipFree(d_slots_);
@@ -94,6 +102,11 @@ hipError_t ResidentEngine::init(int device, size_t arena_bytes, int capacity, co
     if ((e = hipMalloc(&d_lats_, sizeof(PackedLattice) * (size_t)capacity)) != hipSuccess) return e;
     if ((e = hipMalloc(&d_queue_, 256)) != hipSuccess) return e;
     if ((e = hipMalloc(&d_init_idx_, sizeof(int32_t) * (size_t)capacity)) != hipSuccess) return e;
+    for (hipStream_t &a : aux_)
+        if ((e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming)) != hipSuccess) return e;
+    for (hipEvent_t &ev : ev_join_)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
     return hipSuccess;
 }
 
@@ -135,7 +148,7 @@ int ResidentEngine::group_size(int L, int nl) const
     while (cap * 2 <= opt_.max_group) cap *= 2;  // positions are a power of two
     G = std::min(G, cap);
     // keep a slot's LDS modest so that several workgroups share a CU
-    const uint32_t soft = 64u * 1024u;
+    const uint32_t soft = opt_.max_slot_lds ? opt_.max_slot_lds : 32768u * (uint32_t)opt_.threads / 256u;
     while (G > 1 && lds_bytes(L, G, nl) > soft) G /= 2;
     return std::max(G, 1);
 }
@@ -281,10 +294,10 @@ void ResidentEngine::rebuild_slots(uint32_t chunk)
     struct Pending {
         Slot s;
         uint64_t work;
+        uint32_t lds;
     };
     std::vector<Pending> pend;
     slot_lats_.clear();
-    max_lds_ = 0;
     for (auto &kv : groups) {
         const int L = kv.first.L, nl = kv.first.nl;
         const int G = group_size(L, nl);
@@ -307,14 +320,41 @@ void ResidentEngine::rebuild_slots(uint32_t chunk)
                 const Job &j = jobs_[(size_t)id];
                 work += (uint64_t)L * (uint64_t)L * std::min<uint64_t>(ns, j.therm_left + j.meas_left);
             }
-            pend.push_back({s, work});
-            max_lds_ = std::max(max_lds_, lds_bytes(L, gp, nl));
+            pend.push_back({s, work, lds_bytes(L, gp, nl)});
         }
     }
-    // longest slots first: the persistent grid pulls them in this order
-    std::stable_sort(pend.begin(), pend.end(), [](const Pending &a, const Pending &b) { return a.work > b.work; });
+    // launch classes by workgroups per CU (most first); merge the tail into
+    // the last class when there are more than kMaxClasses
+    std::map<uint32_t, int> resident;                  // slot LDS -> workgroups resident on the device
+    std::map<int, int, std::greater<int>> class_of;  // workgroups resident -> class
+    for (const Pending &p : pend) {
+        auto it = resident.find(p.lds);
+        if (it == resident.end()) it = resident.emplace(p.lds, resident_max_active_groups(opt_.threads, p.lds)).first;
+        class_of[it->second] = 0;
+    }
+    int c = 0;
+    for (auto &kv : class_of) kv.second = std::min(c++, kMaxClasses - 1);
+    std::vector<int> cls(pend.size());
+    for (size_t i = 0; i < pend.size(); ++i) cls[i] = class_of[resident[pend[i].lds]];
+    // within a class, longest slots first: the persistent grid pulls them in this order
+    std::vector<size_t> order(pend.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        return cls[a] != cls[b] ? cls[a] < cls[b] : pend[a].work > pend[b].work;
+    });
     slots_.clear();
-    for (auto &p : pend) slots_.push_back(p.s);
+    classes_.clear();
+    for (size_t i : order) {
+        if (classes_.empty() || (int)classes_.size() - 1 != cls[i]) {
+            LaunchClass lc;
+            lc.first = (uint32_t)slots_.size();
+            classes_.push_back(lc);
+        }
+        LaunchClass &lc = classes_.back();
+        lc.count += 1;
+        lc.lds = std::max(lc.lds, pend[i].lds);
+        slots_.push_back(pend[i].s);
+    }
 }
 
 hipError_t ResidentEngine::upload_slots()
@@ -339,12 +379,40 @@ hipError_t ResidentEngine::upload_slots()
                            hipMemcpyHostToDevice)) != hipSuccess)
             return e;
     }
-    grid_ = resident_max_active_groups(opt_.threads, max_lds_);
-    if (grid_ <= 0) {
-        failed_ = "hipOccupancyMaxActiveBlocksPerMultiprocessor";
-        return hipErrorInvalidConfiguration;
+    for (LaunchClass &lc : classes_) {
+        const int resident = resident_max_active_groups(opt_.threads, lc.lds);
+        if (resident <= 0) {
+            failed_ = "hipOccupancyMaxActiveBlocksPerMultiprocessor";
+            return hipErrorInvalidConfiguration;
+        }
+        lc.grid = std::min<int>(resident, (int)lc.count);
     }
-    grid_ = std::min<int>(grid_, (int)slots_.size());
+    return hipSuccess;
+}
+
+hipError_t ResidentEngine::launch_classes(KernelTimer *tm)
+{
+    hipError_t e;
+    const int ev = tm ? tm->begin(kKindResident, stream_) : -1;
+    if (classes_.size() > 1) {
+        if ((e = hipEventRecord(ev_fork_, stream_)) != hipSuccess) return e;
+    }
+    for (size_t c = 0; c < classes_.size(); ++c) {
+        const LaunchClass &lc = classes_[c];
+        hipStream_t s = c == 0 ? stream_ : aux_[c - 1];
+        if (c > 0 && (e = hipStreamWaitEvent(s, ev_fork_, 0)) != hipSuccess) return e;
+        // one queue counter per class, 64 bytes apart
+        if ((e = launch_resident(opt_.threads, d_slots_ + lc.first, lc.count, d_slot_lats_, d_lats_,
+                                 d_queue_ + 16u * c, lc.lds, (uint32_t)lc.grid, s)) != hipSuccess) {
+            failed_ = "launch_resident";
+            return e;
+        }
+        if (c > 0) {
+            if ((e = hipEventRecord(ev_join_[c], s)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(stream_, ev_join_[c], 0)) != hipSuccess) return e;
+        }
+    }
+    if (tm) tm->end(ev, stream_);
     return hipSuccess;
 }
 
@@ -369,12 +437,8 @@ hipError_t ResidentEngine::tick(uint32_t n, uint32_t &advanced, KernelTimer *tm)
         slots_chunk_ = chunk;
     }
     if (slots_.empty()) return hipSuccess;
-    const int ev = tm ? tm->begin(kKindResident, stream_) : -1;
-    e = launch_resident(opt_.threads, d_slots_, (uint32_t)slots_.size(), d_slot_lats_, d_lats_, d_queue_, max_lds_,
-                        (uint32_t)grid_, stream_);
-    if (tm) tm->end(ev, stream_);
-    if (e != hipSuccess) {
-        failed_ = "launch_resident";
+    if ((e = launch_classes(tm)) != hipSuccess) {
+        if (!*failed_) failed_ = "launch_classes";
         return e;
     }
     // mirror the device counters

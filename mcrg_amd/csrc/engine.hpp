@@ -46,6 +46,9 @@ struct EngineOptions {
     int threads = 256;           // workgroup size of the resident kernel
     int sweeps_per_launch = 250;  // chunk length (the most sweeps any lattice runs per launch)
     int max_group = 256;          // upper bound on lattices per slot
+    // LDS a slot of small lattices may take (G halves until it fits);
+    // 0 = 32 KiB per 256 threads, which leaves room for five workgroups per CU.
+    uint32_t max_slot_lds = 0;
     // Time slicing: when nonzero, a slot of G positions of side L runs
     // min(chunk, max(1, quantum / (G L^2))) sweeps per launch, so that every
     // workgroup does about `quantum` site updates; 0 = every slot runs the chunk.
@@ -104,6 +107,21 @@ class ResidentEngine {
     };
     void rebuild_slots(uint32_t chunk);
     uint32_t slot_sweeps(int L, int G, uint32_t chunk) const;
+    hipError_t launch_classes(KernelTimer *tm);
+
+    // Slots whose LDS allows the same number of workgroups per CU form one
+    // launch class; classes launch concurrently on their own streams, each
+    // with its own dynamic LDS size, so that a few large lattices do not cut
+    // the occupancy of every small one.
+    static constexpr int kMaxClasses = 4;
+    struct LaunchClass {
+        uint32_t first = 0, count = 0, lds = 0;
+        int grid = 0;
+    };
+    std::vector<LaunchClass> classes_;
+    hipStream_t aux_[kMaxClasses - 1] = {};
+    hipEvent_t ev_fork_ = nullptr;
+    hipEvent_t ev_join_[kMaxClasses] = {};
     hipError_t upload_slots();
 
     int device_ = 0;
@@ -126,8 +144,6 @@ class ResidentEngine {
     uint32_t slots_chunk_ = 0;
     std::vector<int32_t> pending_init_;
     std::vector<PackedLattice> staging_lats_;
-    uint32_t max_lds_ = 0;
-    int grid_ = 0;
     const char *failed_ = "";
 };
 
