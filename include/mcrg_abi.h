@@ -137,9 +137,13 @@ mcrg_status mcrg_chain_set_state(mcrg_chain ch, uint32_t sweep, uint32_t n_measu
 mcrg_status mcrg_chain_operators(mcrg_chain ch, int64_t out[4]);
 
 /* ------------------------------------------------------------------ batches */
-/* Many independent chains (uneven L and coupling) advanced together by the
- * LDS-resident schedule: bit-packed lattices held in LDS by a persistent grid,
- * one launch per chunk of sweeps.  A job is the reference's whole
+/* Many independent chains (uneven L and coupling) advanced together.  Jobs of
+ * 8 <= L <= 512 run on the LDS-resident schedule: bit-packed lattices held in
+ * LDS by a persistent grid, one launch per chunk of sweeps.  Jobs of L = 2, 4
+ * and L >= 1024 run on the HBM schedule (int8 in device memory): jobs of the
+ * same L, n_levels, n_therm, n_meas and sweeps_per_bin share every launch, on
+ * a second stream that overlaps the resident launches, and long runs replay
+ * captured HIP graphs of 32 sweeps.  A job is the reference's whole
  * calc_critical_exponent chain (src/mcrg.cpp:22-98) for one (L, K, seed):
  * n_therm unmeasured sweeps (equilibrate, src/ising.cpp:14-84) then n_meas
  * measured sweeps.  Every job is bit-identical to the same chain run alone
@@ -149,7 +153,8 @@ mcrg_status mcrg_chain_operators(mcrg_chain ch, int64_t out[4]);
 typedef struct mcrg_batch_s *mcrg_batch;
 
 typedef struct {
-    int32_t L;              /* power of two, 8 <= L <= 512                    */
+    int32_t L;              /* power of two, 2 <= L <= 16384 (server: 8..512);
+                               two or more blockings need L <= 2048            */
     double coupling;        /* J = |K| >= 0                                    */
     uint64_t seed;          /* Philox key                                      */
     int32_t n_therm;        /* unmeasured sweeps                               */
@@ -166,6 +171,8 @@ typedef struct {
     int32_t max_slot_lds;      /* LDS bytes a workgroup of small lattices may take
                                   (lattices per workgroup halve until it fits);
                                   0 = 32 KiB per 256 threads                    */
+    int32_t min_team;          /* lanes per lattice at least (1..threads, power
+                                  of two); 0 = 16                               */
 } mcrg_batch_opts;
 
 mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n_jobs,

@@ -4,6 +4,7 @@
 #include <new>
 #include <string>
 
+#include "hbm_group.hpp"
 #include "runtime.hpp"
 
 namespace mcrg {
@@ -86,6 +87,7 @@ struct mcrg_chain_s {
     void *d_block = nullptr;       // one allocation for spins, level-1, sums, state, records
     uint32_t t = 0;                // host mirror of state[0]
     uint32_t n_measured = 0;       // host mirror of state[1]
+    SweepRunner runner;            // replayed graphs of long sweep runs
 };
 
 extern "C" {
@@ -198,16 +200,8 @@ mcrg_status mcrg_chain_create(mcrg_ctx ctx, const mcrg_chain_desc *desc, mcrg_ch
     }
     ch->ctx = ctx;
     ch->desc = d;
-    const size_t L = (size_t)d.L;
-    const size_t n_spins = L * L;
-    const size_t n_lvl1 = ((L / 2) * (L / 2) + 31) / 32;
-    const size_t rs = record_size((uint32_t)d.n_levels);
-    const size_t o_spins = 0;
-    const size_t o_lvl1 = align256(o_spins + n_spins);
-    const size_t o_sums = align256(o_lvl1 + n_lvl1 * 4);
-    const size_t o_state = align256(o_sums + 4 * 8);
-    const size_t o_rec = align256(o_state + 2 * 4);
-    const size_t o_desc = align256(o_rec + (size_t)d.max_bins * rs * 8);
+    const HbmBlock blk = hbm_block(d.L, d.n_levels, d.max_bins);
+    const size_t o_desc = blk.bytes;
     const size_t total = align256(o_desc + sizeof(HbmLattice));
     hipError_t e = hipMalloc(&ch->d_block, total);
     if (e != hipSuccess) {
@@ -216,19 +210,7 @@ mcrg_status mcrg_chain_create(mcrg_ctx ctx, const mcrg_chain_desc *desc, mcrg_ch
         return MCRG_ERR_OUT_OF_BUDGET;
     }
     char *base = static_cast<char *>(ch->d_block);
-    HbmLattice &h = ch->host;
-    h.spins = reinterpret_cast<int8_t *>(base + o_spins);
-    h.lvl1 = reinterpret_cast<uint32_t *>(base + o_lvl1);
-    h.sums0 = reinterpret_cast<unsigned long long *>(base + o_sums);
-    h.state = reinterpret_cast<uint32_t *>(base + o_state);
-    h.records = reinterpret_cast<int64_t *>(base + o_rec);
-    h.k0 = (uint32_t)d.seed;
-    h.k1 = (uint32_t)(d.seed >> 32);
-    h.acc = make_accept(d.coupling);
-    h.L = d.L;
-    h.n_levels = d.n_levels;
-    h.sweeps_per_bin = d.sweeps_per_bin;
-    h.pad = 0;
+    ch->host = hbm_descriptor(base, blk, d.L, d.n_levels, d.sweeps_per_bin, d.seed, d.coupling);
     ch->d_desc = reinterpret_cast<HbmLattice *>(base + o_desc);
 
     auto fail = [&](hipError_t err, const char *what) {
@@ -289,24 +271,8 @@ mcrg_status mcrg_chain_sweep(mcrg_chain ch, int32_t n_sweeps, int32_t measure)
     }
     MCRG_REQUIRE((uint64_t)ch->t + (uint64_t)n_sweeps <= 0xFFFFFFFFull, "sweep counter overflow");
     MCRG_HIP(hipSetDevice(ch->ctx->device));
-    KernelTimer &tm = ch->ctx->timer;
-    hipStream_t s = ch->stream;
-    for (int32_t i = 0; i < n_sweeps; ++i) {
-        for (uint32_t c = 0; c < 2; ++c) {
-            const int ev = tm.begin(kKindSweep, s);
-            MCRG_HIP(launch_int8_half_sweep(ch->d_desc, 1, d.L, (uint32_t)i, c, s));
-            tm.end(ev, s);
-        }
-        if (measure) {
-            int ev = tm.begin(kKindMeasure0, s);
-            MCRG_HIP(launch_int8_measure0(ch->d_desc, 1, d.L, (uint32_t)i, s));
-            tm.end(ev, s);
-            ev = tm.begin(kKindLevels, s);
-            MCRG_HIP(launch_levels_accumulate(ch->d_desc, 1, (uint32_t)i, (uint32_t)i, s));
-            tm.end(ev, s);
-        }
-    }
-    MCRG_HIP(launch_advance(ch->d_desc, 1, (uint32_t)n_sweeps, measure ? (uint32_t)n_sweeps : 0u, s));
+    KernelTimer *tm = ch->ctx->timer.enabled ? &ch->ctx->timer : nullptr;
+    MCRG_HIP(ch->runner.run(ch->d_desc, 1, d.L, (uint32_t)n_sweeps, measure != 0, ch->stream, tm));
     ch->t += (uint32_t)n_sweeps;
     if (measure) ch->n_measured += (uint32_t)n_sweeps;
     return MCRG_OK;

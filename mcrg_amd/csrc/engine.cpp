@@ -147,6 +147,8 @@ int ResidentEngine::group_size(int L, int nl) const
     int cap = 1;
     while (cap * 2 <= opt_.max_group) cap *= 2;  // positions are a power of two
     G = std::min(G, cap);
+    // a team of at least min_team lanes shares each lattice's measurement
+    G = std::min(G, std::max(1, opt_.threads / std::max(1, opt_.min_team)));
     // keep a slot's LDS modest so that several workgroups share a CU
     const uint32_t soft = opt_.max_slot_lds ? opt_.max_slot_lds : 32768u * (uint32_t)opt_.threads / 256u;
     while (G > 1 && lds_bytes(L, G, nl) > soft) G /= 2;

@@ -46,6 +46,7 @@ struct EngineOptions {
     int threads = 256;           // workgroup size of the resident kernel
     int sweeps_per_launch = 250;  // chunk length (the most sweeps any lattice runs per launch)
     int max_group = 256;          // upper bound on lattices per slot
+    int min_team = 16;            // lanes per lattice at least (measurement parallelism)
     // LDS a slot of small lattices may take (G halves until it fits);
     // 0 = 32 KiB per 256 threads, which leaves room for five workgroups per CU.
     uint32_t max_slot_lds = 0;

@@ -136,7 +136,8 @@ __device__ __forceinline__ void record_pair(uint32_t nl, uint32_t k, uint32_t &x
     }
 }
 
-// Operator value i of sv (see record_pair) from the counts.
+// Operator value i of sv (see record_pair) from the counts.  |value| <= 2 L^2
+// <= 2^19 for L <= 512, so the slot keeps them as int32.
 __device__ __forceinline__ int64_t sv_value(const uint32_t *cntg, uint32_t nw, uint32_t nl, uint32_t L, uint32_t i)
 {
     const uint32_t n_s = 3u * (nl + 1u);
@@ -313,7 +314,8 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
     uint32_t *coins = lds + lay.coin;
     uint32_t *cnt = lds + lay.cnt;
     int64_t *rec = reinterpret_cast<int64_t *>(lds + lay.rec);
-    int64_t *sv = reinterpret_cast<int64_t *>(lds + lay.sv);
+    int32_t *sv = reinterpret_cast<int32_t *>(lds + lay.sv);
+    uint32_t *pairs = lds + lay.pairs;
 
     // ---- load: meta, spins, zero partial records
     for (uint32_t g = tm.g0; g < G; g += tm.gs) {
@@ -348,6 +350,11 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
             for (uint32_t j = u; j < 2u * W; j += T)
                 spins[g * 2u * W + j] = li != 0xFFFFFFFFu ? lats[li].spins[j] : 0u;
             for (uint32_t k = u; k < rs; k += T) rec[g * rs + k] = 0;
+            for (uint32_t k = g * T + u; k < rs; k += G * T) {
+                uint32_t x, y;
+                record_pair(nl, k, x, y);
+                pairs[k] = x | (y << 16);
+            }
         }
     }
     block_sync();
@@ -481,7 +488,8 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
         for (uint32_t g = tm.g0; g < G; g += tm.gs) {
             if (!in_meas(meta + g * kMetaWords, it)) continue;
             for (uint32_t u = tm.u0; u < T; u += tm.us)
-                for (uint32_t i = u; i < lay.n_sv; i += T) sv[g * lay.n_sv + i] = sv_value(cnt + g * cstride, nw, nl, L, i);
+                for (uint32_t i = u; i < lay.n_sv; i += T)
+                    sv[g * lay.n_sv + i] = (int32_t)sv_value(cnt + g * cstride, nw, nl, L, i);
         }
         team_sync(tm);
         for (uint32_t g = tm.g0; g < G; g += tm.gs) {
@@ -490,7 +498,7 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
             const uint32_t q = m[kMetaNMeas] + (it - m[kMetaTherm]);
             const uint32_t spb = m[kMetaSpb];
             const bool new_bin = q % spb == 0u && q > m[kMetaNMeas];
-            const int64_t *svg = sv + g * lay.n_sv;
+            const int32_t *svg = sv + g * lay.n_sv;
             for (uint32_t u = tm.u0; u < T; u += tm.us) {
                 for (uint32_t k = u; k < rs; k += T) {
                     int64_t &r = rec[g * rs + k];
@@ -498,9 +506,8 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
                         lats[m[kMetaIdx]].records[(size_t)((q - 1u) / spb) * rs + k] += r;
                         r = 0;
                     }
-                    uint32_t x, y;
-                    record_pair(nl, k, x, y);
-                    r += svg[x] * svg[y];
+                    const uint32_t xy = pairs[k];
+                    r += (int64_t)svg[xy & 0xFFFFu] * (int64_t)svg[xy >> 16];
                 }
             }
         }

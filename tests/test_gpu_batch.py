@@ -107,7 +107,8 @@ def test_incremental_runs(amd, ctx):
             assert np.array_equal(a.records(j), b.records(j))
 
 
-@pytest.mark.parametrize("bad,msg", [(dict(L=4), "[8, 512]"), (dict(L=1024), "[8, 512]"),
+@pytest.mark.parametrize("bad,msg", [(dict(L=12), "power of two"), (dict(L=32768), "[2, 16384]"),
+                                     (dict(L=4096), "L <= 2048"), (dict(L=4), "n_levels"),
                                      (dict(n_levels=6), "n_levels"), (dict(coupling=-0.1), "coupling"),
                                      (dict(sweeps_per_bin=0), "sweeps_per_bin")])
 def test_invalid_jobs(amd, ctx, bad, msg):
@@ -127,4 +128,23 @@ def test_launch_classes_mixed_lds(amd, ctx, orc):
     print("classes", len(jobs), flush=True)
     with amd.Batch(ctx, jobs, sweeps_per_launch=3) as b:
         b.run_all()
+        _check_vs_oracle(orc, b, jobs)
+
+
+@pytest.mark.parametrize("chunk", [250, 7])
+def test_hbm_groups_bit_exact(amd, ctx, orc, chunk):
+    """L = 2, 4 and L >= 1024 run on the HBM schedule, grouped by shape, beside
+    resident jobs; every job equals the oracle (graph replays included when a
+    chunk covers 32 sweeps or more)."""
+    jobs = [dict(L=1024, coupling=KC, seed=501, n_therm=3, n_meas=40, n_levels=4, sweeps_per_bin=16),
+            dict(L=1024, coupling=0.45, seed=502, n_therm=3, n_meas=40, n_levels=4, sweeps_per_bin=16),
+            dict(L=32, coupling=KC, seed=503, n_therm=3, n_meas=40, n_levels=3, sweeps_per_bin=16),
+            dict(L=4, coupling=KC, seed=504, n_therm=10, n_meas=70, n_levels=1, sweeps_per_bin=8),
+            dict(L=2, coupling=0.3, seed=505, n_therm=0, n_meas=33, n_levels=0, sweeps_per_bin=33),
+            dict(L=1024, coupling=0.44, seed=506, n_therm=35, n_meas=0, n_levels=1, sweeps_per_bin=1),
+            dict(L=4, coupling=0.2, seed=507, n_therm=10, n_meas=70, n_levels=1, sweeps_per_bin=8)]
+    print("hbm groups", chunk, flush=True)
+    with amd.Batch(ctx, jobs, sweeps_per_launch=chunk) as b:
+        b.run_all()
+        b.sync()
         _check_vs_oracle(orc, b, jobs)

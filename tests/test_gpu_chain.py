@@ -164,3 +164,29 @@ def test_record_capacity_error(amd, ctx):
         with pytest.raises(amd.MCRGError) as e:
             ch.sweep(1, measure=True)
         assert e.value.status == amd.ERR_CAPACITY
+
+
+def test_graph_replays_match_oracle(amd, ctx, orc):
+    """Runs of 32 sweeps or more replay a captured graph.  Several replays,
+    with eager reads, eager short runs and a coupling change between them,
+    each checked against the oracle (spins, counters and every record bin)."""
+    L, nl, seed, spb = 64, 3, 4242, 20
+    _say("graph replays", L)
+    s = orc.init_spins(L, seed)
+    rec_ref = np.zeros((8, orc.record_size(nl)), dtype=np.int64)
+    t = q = 0
+    with amd.Chain(ctx, L, KC, seed, n_levels=nl, sweeps_per_bin=spb, max_bins=8) as ch:
+        for n, meas, J in [(64, False, KC), (40, True, KC), (5, True, KC), (70, True, 0.42), (33, False, 0.42)]:
+            ch.set_coupling(J)
+            ch.sweep(n, measure=meas)
+            for _ in range(n):
+                if meas:
+                    s, r = orc.run_chain(L, J, seed, 0, 1, nl, 1, s=s, t0=t)
+                    rec_ref[q // spb] += r[0]
+                    q += 1
+                else:
+                    orc.sweep(L, s, seed, J, t)
+                t += 1
+            assert np.array_equal(ch.spins(), s), (n, meas, J)
+            assert ch.state() == (t, q)
+            assert np.array_equal(ch.records(), rec_ref), (n, meas, J)
