@@ -19,6 +19,13 @@ Measured beside it (same JSON line):
                HIP-event-timed average duration (events on its own stream)
   cpu_baseline the oracle (scalar C) on the host's cores over a sample of the
                manifest (threads over independent replicas)
+  bisection    the critical-coupling bisection (mcrg_run --bisect): per round 16
+               couplings spanning the bracket (start [0.43, 0.45]), 64 lattices
+               of L = 1024 (int8, HBM schedule) and 1024 of L = 128 (bit-packed,
+               LDS-resident) in one batch, 2000 + 10000 sweeps each, NN per site
+               of L = 1024 blocked 4 times against L = 128 blocked once; 8 rounds;
+               reports the final K_c with its jackknife error against the exact
+               0.4406868 and the aggregate updates/ns
   serving      open-loop Poisson job streams (mcrg_amd.workloads.serving_stream:
                L = 2^k, k in 4..9, |K| ~ U[0.42, 0.47], 1e3 thermalisation plus
                U[1e3, 2e4] measured sweeps, blocking to L = 4, seed = job id)
@@ -203,6 +210,20 @@ def cpu_baseline(jobs, sweeps, threads):
                       f"measured sweeps each, {threads} threads over independent replicas: {dt:.1f} s"}
 
 
+def run_bisection(rounds):
+    import subprocess
+    cmd = [os.path.join(ROOT, "mcrg_amd", "mcrg_run"), "--bisect", str(rounds), "--eq", "2000", "--samples",
+           "10000", "--bins", "20", "--quiet"]
+    log("bench: " + " ".join(cmd[1:]))
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=1200)
+    if out.returncode != 0:
+        raise RuntimeError(f"mcrg_run --bisect failed: {out.stderr.strip()}")
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    res["config"] = ("64 x L=1024 int8 + 1024 x L=128 packed per round, 16 couplings, 2000 + 10000 sweeps, "
+                     "NN per site of L=1024 blocked 4x vs L=128 blocked 1x")
+    return res
+
+
 def copy_calibration(n_bytes):
     """A plain elementwise kernel that reads and writes n_bytes each (torch.neg
     on int32), five times: the reference point for the FETCH_SIZE / WRITE_SIZE
@@ -235,6 +256,7 @@ def main():
     ap.add_argument("--serving-drain", type=float, default=120.0,
                     help="seconds after the last arrival to wait for completions")
     ap.add_argument("--serving-quantum", type=int, default=0)
+    ap.add_argument("--bisect-rounds", type=int, default=8, help="rounds of the bisection leg (0 = skip)")
     ap.add_argument("--only", choices=["batched", "plain_l2048"], default=None,
                     help="run one leg alone and print its line to stderr (for profiler runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -247,6 +269,13 @@ def main():
               "independent chains and this build runs on a single device. Run with --gpus 1.",
               file=sys.stderr)
         return 2
+
+    # the bisection driver runs as a child process before this one touches the GPU
+    bisection = None
+    if args.bisect_rounds and not args.only:
+        bisection = run_bisection(args.bisect_rounds)
+        log(f"bench: bisection Kc = {bisection['Jc']:.7f} +- {bisection['Jc_err']:.7f} "
+            f"({bisection['updates_per_ns']:.1f} updates/ns, {bisection['seconds']:.1f} s)")
 
     import mcrg_amd
     from mcrg_amd import workloads
@@ -323,6 +352,7 @@ def main():
         "plain_l2048": plain_2048,
         "batched_over_plain_mixed": (value / plain_mixed) if plain_mixed else None,
         "serving": serving,
+        "bisection": bisection,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -2,6 +2,7 @@
 #include "mcrg.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <iomanip>
 #include <mutex>
@@ -336,8 +337,9 @@ std::vector<int64_t> MonteCarloRenormalizationGroup::run_chains(int N, double K,
     n_bins = std::max(1, (max_loc + spb - 1) / spb);
     std::vector<int64_t> total((size_t)n_bins * rs, 0);
     const double J = -K;
-    if (N >= 8 && N <= 512) {
-        // replicas advance together on the LDS-resident schedule
+    {
+        // replicas advance together in one batch (LDS-resident for 8 <= N <= 512,
+        // one HBM-schedule group otherwise)
         std::vector<mcrg_job_desc> jobs((size_t)R);
         for (int r = 0; r < R; ++r) {
             mcrg_job_desc &j = jobs[(size_t)r];
@@ -360,18 +362,135 @@ std::vector<int64_t> MonteCarloRenormalizationGroup::run_chains(int N, double K,
             for (size_t k = 0; k < rec.size(); ++k) total[k] += rec[k];
         }
         mcrg_batch_destroy(b);
-    } else {
-        for (int r = 0; r < R; ++r) {
-            mcrg_chain ch = new_chain(N, J, mcrg_cpp::next_seed(), n_levels, spb, n_bins);
-            check(mcrg_chain_sweep(ch, n_samples_eq, 0));
-            check(mcrg_chain_sweep(ch, n_loc[(size_t)r], 1));
-            std::vector<int64_t> rec((size_t)n_bins * rs);
-            check(mcrg_chain_get_records(ch, rec.data(), (int64_t)rec.size()));
-            mcrg_chain_destroy(ch);
-            for (size_t k = 0; k < rec.size(); ++k) total[k] += rec[k];
-        }
     }
     return total;
+}
+
+namespace {
+// Weighted straight line D = a + b J through the points with finite weights;
+// returns the crossing -a / b (NaN when undefined).
+double crossing(const std::vector<double> &J, const std::vector<double> &D, const std::vector<double> &w)
+{
+    double S = 0, Sx = 0, Sy = 0, Sxx = 0, Sxy = 0;
+    for (size_t i = 0; i < J.size(); ++i) {
+        S += w[i];
+        Sx += w[i] * J[i];
+        Sy += w[i] * D[i];
+        Sxx += w[i] * J[i] * J[i];
+        Sxy += w[i] * J[i] * D[i];
+    }
+    const double det = S * Sxx - Sx * Sx;
+    if (det == 0.0) return std::nan("");
+    const double b = (S * Sxy - Sx * Sy) / det, a = (Sy - b * Sx) / S;
+    return b != 0.0 ? -a / b : std::nan("");
+}
+}  // namespace
+
+MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect_critical_point(
+    int n_rounds, int n_k, double J_lo, double J_hi, int n_large, int L_large, int n_small, int L_small,
+    int n_samples_eq, int n_samples)
+{
+    if (n_rounds < 1 || n_k < 2 || !(J_hi > J_lo) || !(J_lo > 0.0))
+        throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "need n_rounds >= 1, n_k >= 2 and 0 < J_lo < J_hi");
+    if (L_small < 8 || L_large <= L_small || (L_large % L_small) != 0 || n_large % n_k || n_small % n_k ||
+        n_large < n_k || n_small < n_k || n_samples < 2)
+        throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG,
+                              "need L_large = L_small * 2^k, L_small >= 8, and n_large, n_small multiples of n_k");
+    int k = 0;
+    while ((L_small << k) < L_large) ++k;
+    if ((L_small << k) != L_large) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "L_large / L_small must be 2^k");
+    const int nl_small = 1, nl_large = k + nl_small;  // matched at m = 0 (L_small) and m = 1 (L_small / 2)
+    const int spb = std::max(1, n_samples / std::max(2, bins_));
+    const int nb = (n_samples + spb - 1) / spb;
+    const int rs_l = mcrg_record_size(nl_large), rs_s = mcrg_record_size(nl_small);
+    const int per_l = n_large / n_k, per_s = n_small / n_k;
+    Bisection out;
+    const auto t0 = std::chrono::steady_clock::now();
+    double lo = J_lo, hi = J_hi;
+    for (int round = 0; round < n_rounds; ++round) {
+        BisectionRound br;
+        br.J_lo = lo;
+        br.J_hi = hi;
+        std::vector<mcrg_job_desc> jobs;
+        for (int q = 0; q < n_k; ++q) {
+            const double J = lo + (hi - lo) * (q + 0.5) / n_k;
+            br.J.push_back(J);
+            for (int r = 0; r < per_l; ++r)
+                jobs.push_back({L_large, J, mcrg_cpp::next_seed(), n_samples_eq, n_samples, nl_large, spb});
+            for (int r = 0; r < per_s; ++r)
+                jobs.push_back({L_small, J, mcrg_cpp::next_seed(), n_samples_eq, n_samples, nl_small, spb});
+        }
+        mcrg_batch b = nullptr;
+        check(mcrg_batch_create(mcrg_cpp::context(), jobs.data(), (int32_t)jobs.size(), nullptr, &b));
+        int32_t left = 1;
+        while (left > 0) check(mcrg_batch_run(b, 1 << 30, &left));
+        // per coupling and bin: NN sums at the matched levels and sample counts
+        std::vector<double> SL((size_t)n_k * nb * 2, 0.0), SS((size_t)n_k * nb * 2, 0.0);
+        std::vector<double> NL((size_t)n_k * nb, 0.0), NS((size_t)n_k * nb, 0.0);
+        std::vector<int64_t> rec;
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            const int q = (int)(j / (size_t)(per_l + per_s));
+            const bool large = (int)(j % (size_t)(per_l + per_s)) < per_l;
+            const int nl = large ? nl_large : nl_small, rs = large ? rs_l : rs_s;
+            rec.assign((size_t)nb * rs, 0);
+            check(mcrg_batch_get_records(b, (int32_t)j, rec.data(), (int64_t)rec.size()));
+            for (int bin = 0; bin < nb; ++bin) {
+                const int64_t *r = rec.data() + (size_t)bin * rs;
+                for (int m = 0; m < 2; ++m) {
+                    const int lvl = large ? k + m : m;
+                    (large ? SL : SS)[((size_t)q * nb + bin) * 2 + m] += (double)r[lvl * 3];
+                }
+                (large ? NL : NS)[(size_t)q * nb + bin] += (double)r[12 * (nl + 1) + 10 * nl + 3];
+            }
+            out.site_updates += (double)jobs[j].L * jobs[j].L * (double)(n_samples_eq + n_samples);
+        }
+        mcrg_batch_destroy(b);
+        // D per coupling (NN per site at the matched size), all bins and each bin left out
+        auto D_of = [&](int q, int m, int skip) {
+            double sl = 0, ss = 0, nl_ = 0, ns_ = 0;
+            for (int bin = 0; bin < nb; ++bin) {
+                if (bin == skip) continue;
+                sl += SL[((size_t)q * nb + bin) * 2 + m];
+                ss += SS[((size_t)q * nb + bin) * 2 + m];
+                nl_ += NL[(size_t)q * nb + bin];
+                ns_ += NS[(size_t)q * nb + bin];
+            }
+            const double sites = (double)(L_small >> m) * (double)(L_small >> m);
+            return (sl / nl_ - ss / ns_) / sites;
+        };
+        const double jk = (double)(nb - 1) / nb;
+        std::vector<double> w((size_t)n_k);
+        for (int q = 0; q < n_k; ++q) {
+            const double d = D_of(q, 1, -1);
+            double var = 0.0;
+            for (int bin = 0; bin < nb; ++bin) var += (D_of(q, 1, bin) - d) * (D_of(q, 1, bin) - d);
+            var *= jk;
+            br.D.push_back(d);
+            br.D_err.push_back(std::sqrt(var));
+            br.D0.push_back(D_of(q, 0, -1));
+            w[(size_t)q] = var > 0.0 ? 1.0 / var : 0.0;
+        }
+        br.Jc = crossing(br.J, br.D, w);
+        double var = 0.0;
+        std::vector<double> Dq((size_t)n_k);
+        for (int bin = 0; bin < nb; ++bin) {
+            for (int q = 0; q < n_k; ++q) Dq[(size_t)q] = D_of(q, 1, bin);
+            const double c = crossing(br.J, Dq, w);
+            var += (c - br.Jc) * (c - br.Jc);
+        }
+        br.Jc_err = std::sqrt(jk * var);
+        if (!quiet_)
+            std::printf("round %i: J in [%.7f, %.7f] -> Jc = %.7f +- %.7f\n", round + 1, lo, hi, br.Jc, br.Jc_err);
+        out.rounds.push_back(br);
+        if (!std::isfinite(br.Jc)) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "no crossing: D(J) has no slope");
+        const double half = std::max(3.0 * br.Jc_err, (hi - lo) / 16.0);
+        lo = std::max(1e-6, br.Jc - half);
+        hi = br.Jc + half;
+    }
+    out.Jc = out.rounds.back().Jc;
+    out.Jc_err = out.rounds.back().Jc_err;
+    out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return out;
 }
 
 void MonteCarloRenormalizationGroup::calc_critical_exponent(int n_samples_eq, int n_samples, int N, double K)

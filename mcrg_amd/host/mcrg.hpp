@@ -155,6 +155,27 @@ class MonteCarloRenormalizationGroup {
     // per-level estimates of the last approx_critical_point
     const std::vector<double> &last_kc() const { return last_kc_; }
 
+    // K-bisection for the critical coupling with lattices of two sizes run
+    // concurrently (no reference counterpart; it batches the matching idea of
+    // approx_critical_point, src/mcrg.cpp:191-310).  Each round runs, for n_k
+    // couplings J spanning [J_lo, J_hi], n_large / n_k lattices of L_large and
+    // n_small / n_k of L_small in one batch, and compares the nearest-neighbour
+    // correlation per site of L_large blocked k + m times with L_small blocked
+    // m times (k = log2(L_large / L_small), m = 0, 1: same final size).  D(J) =
+    // large - small changes sign at the critical coupling; a weighted straight
+    // line through D(J) gives the crossing J0 and its delete-one-bin jackknife
+    // error, and the next bracket is J0 +- max(3 err, width / 8).  J = |K|.
+    struct BisectionRound {
+        double J_lo = 0, J_hi = 0, Jc = 0, Jc_err = 0;
+        std::vector<double> J, D, D_err, D0;
+    };
+    struct Bisection {
+        std::vector<BisectionRound> rounds;
+        double Jc = 0, Jc_err = 0, seconds = 0, site_updates = 0;
+    };
+    Bisection bisect_critical_point(int n_rounds, int n_k, double J_lo, double J_hi, int n_large, int L_large,
+                                    int n_small, int L_small, int n_samples_eq, int n_samples);
+
   private:
     double approx_critical_point(int n_samples_eq, int n_samples, int L, double K);
     // records of `replicas` chains of side N at coupling K (summed per bin)

@@ -18,6 +18,8 @@ static void usage()
         "usage: mcrg_run [--N n] [--eq n] [--samples n] [--K k ...] [--replicas r] [--ops 2|3]\n"
         "                [--bins b] [--levels l] [--seed s] [--device d] [--outdir dir] [--quiet]\n"
         "                [--locate iterations --L l --K0 k]\n"
+        "                [--bisect rounds --n-k n --J-lo j --J-hi j --n-large n --L-large l\n"
+        "                 --n-small n --L-small l]   (prints one JSON line)\n"
         "K values are in the reference's sign convention (K < 0 ferromagnetic); 'Kc' means\n"
         "-ln(1+sqrt(2))/2.\n");
 }
@@ -26,6 +28,8 @@ int main(int argc, char **argv)
 {
     int b = 2, N = 128, n_eq = 10000, n = 1000000, replicas = 1, ops = 2, bins = 50, levels = -1, device = 0;
     int locate = 0, Lloc = 16;
+    int bisect = 0, n_k = 16, n_large = 64, L_large = 1024, n_small = 1024, L_small = 128;
+    double J_lo = 0.43, J_hi = 0.45;
     double K0 = -0.44;
     unsigned long long seed = 1;
     bool quiet = false;
@@ -57,6 +61,14 @@ int main(int argc, char **argv)
         else if (a == "--quiet") quiet = true;
         else if (a == "--locate") locate = std::atoi(next());
         else if (a == "--L") Lloc = std::atoi(next());
+        else if (a == "--bisect") bisect = std::atoi(next());
+        else if (a == "--n-k") n_k = std::atoi(next());
+        else if (a == "--J-lo") J_lo = std::atof(next());
+        else if (a == "--J-hi") J_hi = std::atof(next());
+        else if (a == "--n-large") n_large = std::atoi(next());
+        else if (a == "--L-large") L_large = std::atoi(next());
+        else if (a == "--n-small") n_small = std::atoi(next());
+        else if (a == "--L-small") L_small = std::atoi(next());
         else if (a == "--K0") {
             const std::string v = next();
             K0 = v == "Kc" ? Kc : std::atof(v.c_str());
@@ -76,7 +88,20 @@ int main(int argc, char **argv)
         MCRG.set_levels(levels);
         MCRG.set_output_dir(outdir);
         MCRG.set_quiet(quiet);
-        if (locate > 0) {
+        if (bisect > 0) {
+            const auto r = MCRG.bisect_critical_point(bisect, n_k, J_lo, J_hi, n_large, L_large, n_small, L_small,
+                                                      n_eq, n);
+            const double exact = std::log(1.0 + std::sqrt(2.0)) / 2.0;
+            std::printf("{\"Jc\": %.10f, \"Jc_err\": %.10f, \"exact\": %.10f, \"deviation_sigma\": %.3f, "
+                        "\"rounds\": %i, \"seconds\": %.3f, \"site_updates\": %.6e, \"updates_per_ns\": %.3f, "
+                        "\"round_Jc\": [",
+                        r.Jc, r.Jc_err, exact, r.Jc_err > 0 ? (r.Jc - exact) / r.Jc_err : 0.0, (int)r.rounds.size(),
+                        r.seconds, r.site_updates, r.site_updates / (r.seconds * 1e9));
+            for (size_t i = 0; i < r.rounds.size(); ++i)
+                std::printf("%s[%.10f, %.10f, %.10f, %.10f]", i ? ", " : "", r.rounds[i].J_lo, r.rounds[i].J_hi,
+                            r.rounds[i].Jc, r.rounds[i].Jc_err);
+            std::printf("]}\n");
+        } else if (locate > 0) {
             MCRG.locate_critical_point(locate, n_eq, n, Lloc, K0);
         } else {
             for (double K : Ks) MCRG.calc_critical_exponent(n_eq, n, N, K);

@@ -112,3 +112,20 @@ def test_cli_rejects_bad_coupling(tmp_path):
     out = subprocess.run([RUN, "--N", "16", "--K", "0.3", "--eq", "1", "--samples", "10", "--outdir",
                           str(tmp_path)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert out.returncode != 0 and "ferromagnetic" in out.stderr
+
+
+def test_bisection_driver(tmp_path):
+    """K-bisection with two lattice sizes in one batch (L = 64 on the LDS-resident
+    schedule, L = 16 beside it): a crossing inside the first bracket, each
+    round's bracket centred on the previous crossing."""
+    out = _run([RUN, "--bisect", "2", "--n-k", "8", "--n-large", "16", "--L-large", "64", "--n-small", "64",
+                "--L-small", "16", "--eq", "500", "--samples", "4000", "--bins", "20", "--quiet"], tmp_path)
+    r = json.loads(out.strip().splitlines()[-1])
+    assert r["rounds"] == 2 and len(r["round_Jc"]) == 2
+    (lo1, hi1, jc1, e1), (lo2, hi2, jc2, e2) = r["round_Jc"]
+    assert (lo1, hi1) == (0.43, 0.45)
+    assert lo1 < jc1 < hi1 and e1 > 0
+    assert (lo2 + hi2) / 2 == pytest.approx(jc1, abs=1e-9)
+    assert r["Jc"] == jc2 and r["Jc_err"] == e2
+    assert abs(r["Jc"] - r["exact"]) < 0.01
+    assert r["site_updates"] == 2 * (16 * 64 * 64 + 64 * 16 * 16) * 4500
