@@ -41,6 +41,13 @@ enum {
 
 enum { MCRG_LAYOUT_INT8 = 0, MCRG_LAYOUT_PACKED = 1 };
 
+/* Samplers: one "sweep" is a red/black checkerboard Metropolis pass, or one
+ * Swendsen-Wang pass (every aligned pair bonded with probability 1 - exp(-2J),
+ * every cluster flipped with probability 1/2), the all-clusters form of the
+ * reference's Wolff update (src/ising.cpp:87-149).  Swendsen-Wang runs on the
+ * HBM schedule (int8). */
+enum { MCRG_SAMPLER_METROPOLIS = 0, MCRG_SAMPLER_SWENDSEN_WANG = 1 };
+
 typedef struct mcrg_ctx_s *mcrg_ctx;
 typedef struct mcrg_chain_s *mcrg_chain;
 
@@ -52,6 +59,11 @@ const char *mcrg_last_error(void);
  * "always flip").  Replaces P_ = 1 - exp(2K) of IsingModel(double K),
  * src/ising.cpp:9. */
 mcrg_status mcrg_thresholds(double coupling, uint64_t thr[2]);
+
+/* Swendsen-Wang bond threshold floor((1 - exp(-2J)) 2^32): an aligned pair is
+ * bonded when its 32-bit uniform is below it.  The reference's Wolff step adds
+ * an aligned neighbour with P_ = 1 - exp(2K) (src/ising.cpp:9,134). */
+mcrg_status mcrg_bond_threshold(double coupling, uint32_t *thr);
 
 /* int64 entries in one per-bin record for n_levels blockings:
  * 22 n_levels + 16.  Layout (l = 0..n_levels, a, b in {NN, NNN, P}):
@@ -92,6 +104,7 @@ typedef struct {
     int32_t layout;         /* MCRG_LAYOUT_INT8 or MCRG_LAYOUT_PACKED        */
     int32_t sweeps_per_bin; /* measured sweeps per record bin (>= 1)         */
     int32_t max_bins;       /* record bins allocated (>= 1)                  */
+    int32_t sampler;        /* MCRG_SAMPLER_* (0 = Metropolis)               */
 } mcrg_chain_desc;
 
 /* Allocates the chain on the device and draws the Philox +-1 start.
@@ -138,10 +151,11 @@ mcrg_status mcrg_chain_operators(mcrg_chain ch, int64_t out[4]);
 
 /* ------------------------------------------------------------------ batches */
 /* Many independent chains (uneven L and coupling) advanced together.  Jobs of
- * 8 <= L <= 512 run on the LDS-resident schedule: bit-packed lattices held in
- * LDS by a persistent grid, one launch per chunk of sweeps.  Jobs of L = 2, 4
- * and L >= 1024 run on the HBM schedule (int8 in device memory): jobs of the
- * same L, n_levels, n_therm, n_meas and sweeps_per_bin share every launch, on
+ * 8 <= L <= 512 with the Metropolis sampler run on the LDS-resident schedule:
+ * bit-packed lattices held in LDS by a persistent grid, one launch per chunk of
+ * sweeps.  Jobs of L = 2, 4 and L >= 1024, and Swendsen-Wang jobs, run on the
+ * HBM schedule (int8 in device memory): jobs of the same L, n_levels, n_therm,
+ * n_meas, sweeps_per_bin and sampler share every launch, on
  * a second stream that overlaps the resident launches, and long runs replay
  * captured HIP graphs of 32 sweeps.  A job is the reference's whole
  * calc_critical_exponent chain (src/mcrg.cpp:22-98) for one (L, K, seed):
@@ -162,6 +176,8 @@ typedef struct {
     int32_t n_levels;       /* blockings per sample, <= log2(L) - 1            */
     int32_t sweeps_per_bin; /* measured sweeps per record bin (>= 1); the job
                                has ceil(n_meas / sweeps_per_bin) bins         */
+    int32_t sampler;        /* MCRG_SAMPLER_* (0 = Metropolis); Swendsen-Wang
+                               jobs run on the HBM schedule at any L          */
 } mcrg_job_desc;
 
 typedef struct {

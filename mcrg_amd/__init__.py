@@ -27,6 +27,8 @@ ERR_NOT_FOUND = -5
 ERR_CAPACITY = -6
 LAYOUT_INT8 = 0
 LAYOUT_PACKED = 1
+SAMPLER_METROPOLIS = 0
+SAMPLER_SWENDSEN_WANG = 1
 JOB_QUEUED = 0
 JOB_RUNNING = 1
 JOB_DONE = 2
@@ -42,13 +44,13 @@ class MCRGError(RuntimeError):
 class ChainDesc(ctypes.Structure):
     _fields_ = [("L", ctypes.c_int32), ("coupling", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("n_levels", ctypes.c_int32), ("layout", ctypes.c_int32),
-                ("sweeps_per_bin", ctypes.c_int32), ("max_bins", ctypes.c_int32)]
+                ("sweeps_per_bin", ctypes.c_int32), ("max_bins", ctypes.c_int32), ("sampler", ctypes.c_int32)]
 
 
 class JobDesc(ctypes.Structure):
     _fields_ = [("L", ctypes.c_int32), ("coupling", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("n_therm", ctypes.c_int32), ("n_meas", ctypes.c_int32), ("n_levels", ctypes.c_int32),
-                ("sweeps_per_bin", ctypes.c_int32)]
+                ("sweeps_per_bin", ctypes.c_int32), ("sampler", ctypes.c_int32)]
 
 
 class BatchOpts(ctypes.Structure):
@@ -92,6 +94,7 @@ _SIGS = {
     "mcrg_last_error": (_c.c_char_p, []),
     "mcrg_thresholds": (_c.c_int32, [_c.c_double, _c.POINTER(_c.c_uint64)]),
     "mcrg_record_size": (_c.c_int32, [_c.c_int32]),
+    "mcrg_bond_threshold": (_c.c_int32, [_c.c_double, _c.POINTER(_c.c_uint32)]),
     "mcrg_ctx_create": (_c.c_int32, [_c.c_int32, _c.POINTER(_p)]),
     "mcrg_ctx_destroy": (_c.c_int32, [_p]),
     "mcrg_ctx_set_timing": (_c.c_int32, [_p, _c.c_int32]),
@@ -161,6 +164,12 @@ def thresholds(coupling):
     thr = (ctypes.c_uint64 * 2)()
     check(_lib.mcrg_thresholds(float(coupling), thr))
     return np.array(list(thr), dtype=np.uint64)
+
+
+def bond_threshold(coupling):
+    thr = ctypes.c_uint32()
+    check(_lib.mcrg_bond_threshold(float(coupling), ctypes.byref(thr)))
+    return thr.value
 
 
 def record_size(n_levels):
@@ -241,8 +250,8 @@ class Chain:
     """One Markov chain on the device (mcrg_chain)."""
 
     def __init__(self, ctx, L, coupling, seed, n_levels=0, layout=LAYOUT_INT8,
-                 sweeps_per_bin=1000, max_bins=1):
-        d = ChainDesc(L, float(coupling), seed, n_levels, layout, sweeps_per_bin, max_bins)
+                 sweeps_per_bin=1000, max_bins=1, sampler=SAMPLER_METROPOLIS):
+        d = ChainDesc(L, float(coupling), seed, n_levels, layout, sweeps_per_bin, max_bins, sampler)
         h = ctypes.c_void_p()
         check(_lib.mcrg_chain_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)))
         self.handle = h
@@ -314,7 +323,7 @@ class Batch:
         arr = (JobDesc * len(jobs))()
         for i, j in enumerate(jobs):
             arr[i] = JobDesc(j["L"], float(j["coupling"]), j["seed"], j.get("n_therm", 0), j.get("n_meas", 0),
-                             j.get("n_levels", 0), j.get("sweeps_per_bin", 1))
+                             j.get("n_levels", 0), j.get("sweeps_per_bin", 1), j.get("sampler", 0))
         opts = BatchOpts(threads, sweeps_per_launch, max_group, max_slot_lds, min_team)
         h = ctypes.c_void_p()
         check(_lib.mcrg_batch_create(ctx.handle, arr, len(jobs), ctypes.byref(opts), ctypes.byref(h)))
@@ -381,7 +390,7 @@ class Server:
 
     def submit(self, job):
         d = JobDesc(job["L"], float(job["coupling"]), job["seed"], job.get("n_therm", 0), job.get("n_meas", 0),
-                    job.get("n_levels", 0), job.get("sweeps_per_bin", 1))
+                    job.get("n_levels", 0), job.get("sweeps_per_bin", 1), job.get("sampler", 0))
         jid = ctypes.c_uint64()
         check(_lib.mcrg_server_submit(self.handle, ctypes.byref(d), ctypes.byref(jid)))
         self.jobs[jid.value] = dict(job)

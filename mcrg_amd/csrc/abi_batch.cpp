@@ -1,6 +1,7 @@
-// C-ABI: batches of independent chains.  Lattices of 8 <= L <= 512 run on the
-// LDS-resident schedule (bit-packed, one persistent launch per chunk); the
-// others (L = 2, 4 and L >= 1024) run on the HBM schedule (int8), grouped by
+// C-ABI: batches of independent chains.  Metropolis lattices of 8 <= L <= 512
+// run on the LDS-resident schedule (bit-packed, one persistent launch per
+// chunk); the others (L = 2, 4, L >= 1024, and Swendsen-Wang jobs) run on the
+// HBM schedule (int8), grouped by
 // shape so that each group advances through shared launches, on a second
 // stream that overlaps the resident launches.
 #include <map>
@@ -22,7 +23,10 @@ struct JobRef {
     int index;  // engine id, or member of the group
 };
 
-bool resident_job(const JobSpec &s) { return s.L >= kResidentMinL && s.L <= kResidentMaxL; }
+bool resident_job(const JobSpec &s)
+{
+    return s.sampler == MCRG_SAMPLER_METROPOLIS && s.L >= kResidentMinL && s.L <= kResidentMaxL;
+}
 }  // namespace
 
 struct mcrg_batch_s {
@@ -91,6 +95,7 @@ mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n
         s.n_meas = (uint32_t)d.n_meas;
         s.n_levels = d.n_levels;
         s.sweeps_per_bin = d.sweeps_per_bin;
+        s.sampler = d.sampler;
         const std::string why = resident_job(s) ? ResidentEngine::check(s) : HbmGroup::check(s);
         if (!why.empty()) {
             set_error("job " + std::to_string(i) + ": " + why);
@@ -126,7 +131,7 @@ mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n
     if (e != hipSuccess) return fail(e, "batch allocation");
     b->refs.resize((size_t)n_jobs);
     // HBM-schedule jobs: one group per shape, members in job order
-    std::map<std::tuple<int, int, uint32_t, uint32_t, int>, int> group_of;
+    std::map<std::tuple<int, int, uint32_t, uint32_t, int, int>, int> group_of;
     std::vector<std::vector<int>> members;
     for (int32_t i = 0; i < n_jobs; ++i) {
         const JobSpec &s = specs[(size_t)i];
@@ -140,7 +145,7 @@ mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n
             b->refs[(size_t)i] = {-1, id};
             continue;
         }
-        const auto key = std::make_tuple(s.L, s.n_levels, s.n_therm, s.n_meas, s.sweeps_per_bin);
+        const auto key = std::make_tuple(s.L, s.n_levels, s.n_therm, s.n_meas, s.sweeps_per_bin, s.sampler);
         auto it = group_of.find(key);
         if (it == group_of.end()) {
             it = group_of.emplace(key, (int)members.size()).first;

@@ -25,6 +25,14 @@ Accept make_accept(double coupling)
     return a;
 }
 
+uint32_t bond_threshold(double coupling)
+{
+    double v = std::floor(-std::expm1(-2.0 * coupling) * 4294967296.0);
+    if (v < 0.0) v = 0.0;
+    if (v > 4294967295.0) v = 4294967295.0;
+    return (uint32_t)v;
+}
+
 hipEvent_t KernelTimer::get()
 {
     if (!pool.empty()) {
@@ -111,6 +119,15 @@ mcrg_status mcrg_thresholds(double coupling, uint64_t thr[2])
     return MCRG_OK;
 }
 
+mcrg_status mcrg_bond_threshold(double coupling, uint32_t *thr)
+{
+    MCRG_REQUIRE(thr != nullptr, "thr is null");
+    MCRG_REQUIRE(std::isfinite(coupling) && coupling >= 0.0,
+                 "coupling J = |K| must be finite and >= 0 (ferromagnetic; reference K = -J)");
+    *thr = bond_threshold(coupling);
+    return MCRG_OK;
+}
+
 int32_t mcrg_record_size(int32_t n_levels)
 {
     if (n_levels < 0 || n_levels > (int32_t)kMaxLevels) return -1;
@@ -191,6 +208,8 @@ mcrg_status mcrg_chain_create(mcrg_ctx ctx, const mcrg_chain_desc *desc, mcrg_ch
                  "two or more blockings on the HBM schedule need L <= 2048");
     MCRG_REQUIRE(d.sweeps_per_bin >= 1 && d.max_bins >= 1, "sweeps_per_bin and max_bins must be >= 1");
     MCRG_REQUIRE(d.layout == MCRG_LAYOUT_INT8, "only MCRG_LAYOUT_INT8 chains run on this schedule");
+    MCRG_REQUIRE(d.sampler == MCRG_SAMPLER_METROPOLIS || d.sampler == MCRG_SAMPLER_SWENDSEN_WANG,
+                 "sampler must be MCRG_SAMPLER_METROPOLIS or MCRG_SAMPLER_SWENDSEN_WANG");
     MCRG_HIP(hipSetDevice(ctx->device));
 
     mcrg_chain ch = new (std::nothrow) mcrg_chain_s;
@@ -200,7 +219,7 @@ mcrg_status mcrg_chain_create(mcrg_ctx ctx, const mcrg_chain_desc *desc, mcrg_ch
     }
     ch->ctx = ctx;
     ch->desc = d;
-    const HbmBlock blk = hbm_block(d.L, d.n_levels, d.max_bins);
+    const HbmBlock blk = hbm_block(d.L, d.n_levels, d.max_bins, d.sampler);
     const size_t o_desc = blk.bytes;
     const size_t total = align256(o_desc + sizeof(HbmLattice));
     hipError_t e = hipMalloc(&ch->d_block, total);
@@ -210,7 +229,7 @@ mcrg_status mcrg_chain_create(mcrg_ctx ctx, const mcrg_chain_desc *desc, mcrg_ch
         return MCRG_ERR_OUT_OF_BUDGET;
     }
     char *base = static_cast<char *>(ch->d_block);
-    ch->host = hbm_descriptor(base, blk, d.L, d.n_levels, d.sweeps_per_bin, d.seed, d.coupling);
+    ch->host = hbm_descriptor(base, blk, d.L, d.n_levels, d.sweeps_per_bin, d.seed, d.coupling, d.sampler);
     ch->d_desc = reinterpret_cast<HbmLattice *>(base + o_desc);
 
     auto fail = [&](hipError_t err, const char *what) {
@@ -250,6 +269,7 @@ mcrg_status mcrg_chain_set_coupling(mcrg_chain ch, double coupling)
     MCRG_HIP(hipSetDevice(ch->ctx->device));
     ch->desc.coupling = coupling;
     ch->host.acc = make_accept(coupling);
+    ch->host.bond_thr = bond_threshold(coupling);
     MCRG_HIP(hipMemcpyAsync(ch->d_desc, &ch->host, sizeof(HbmLattice), hipMemcpyHostToDevice, ch->stream));
     // the host descriptor may change again before the copy runs
     MCRG_HIP(hipStreamSynchronize(ch->stream));
@@ -272,7 +292,7 @@ mcrg_status mcrg_chain_sweep(mcrg_chain ch, int32_t n_sweeps, int32_t measure)
     MCRG_REQUIRE((uint64_t)ch->t + (uint64_t)n_sweeps <= 0xFFFFFFFFull, "sweep counter overflow");
     MCRG_HIP(hipSetDevice(ch->ctx->device));
     KernelTimer *tm = ch->ctx->timer.enabled ? &ch->ctx->timer : nullptr;
-    MCRG_HIP(ch->runner.run(ch->d_desc, 1, d.L, (uint32_t)n_sweeps, measure != 0, ch->stream, tm));
+    MCRG_HIP(ch->runner.run(ch->d_desc, 1, d.L, d.sampler, (uint32_t)n_sweeps, measure != 0, ch->stream, tm));
     ch->t += (uint32_t)n_sweeps;
     if (measure) ch->n_measured += (uint32_t)n_sweeps;
     return MCRG_OK;

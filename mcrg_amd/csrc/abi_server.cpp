@@ -265,6 +265,7 @@ mcrg_status mcrg_server_submit(mcrg_server s, const mcrg_job_desc *job, uint64_t
     spec.n_meas = (uint32_t)job->n_meas;
     spec.n_levels = job->n_levels;
     spec.sweeps_per_bin = job->sweeps_per_bin;
+    spec.sampler = job->sampler;
     const std::string why = ResidentEngine::check(spec);
     if (!why.empty()) {
         set_error(why);

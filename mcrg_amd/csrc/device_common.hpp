@@ -20,6 +20,8 @@ namespace mcrg {
 constexpr uint32_t kTagSweep = 1u;
 constexpr uint32_t kTagTie = 2u;
 constexpr uint32_t kTagInit = 3u;
+constexpr uint32_t kTagSwBond = 4u;
+constexpr uint32_t kTagSwFlip = 5u;
 
 struct u32x4 {
     uint32_t v0, v1, v2, v3;

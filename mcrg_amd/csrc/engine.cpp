@@ -118,6 +118,7 @@ std::string ResidentEngine::check(const JobSpec &j)
     if (j.n_levels < 0 || j.n_levels > ilog2(j.L) - 1) return "n_levels must be in [0, log2(L) - 1]";
     if (j.sweeps_per_bin < 1) return "sweeps_per_bin must be >= 1";
     if ((uint64_t)j.n_therm + (uint64_t)j.n_meas > 0xFFFFFFFFull) return "too many sweeps";
+    if (j.sampler != MCRG_SAMPLER_METROPOLIS) return "the LDS-resident schedule runs the Metropolis sampler only";
     return std::string();
 }
 

@@ -22,6 +22,7 @@ struct JobSpec {
     uint32_t n_therm = 0, n_meas = 0;
     int n_levels = 0;
     int sweeps_per_bin = 1;
+    int sampler = 0;  // MCRG_SAMPLER_*
 };
 
 // First-fit allocator over one device allocation (offsets in bytes).

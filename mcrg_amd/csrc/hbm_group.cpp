@@ -8,7 +8,7 @@ namespace mcrg {
 
 static size_t align256(size_t v) { return (v + 255u) & ~size_t(255u); }
 
-HbmBlock hbm_block(int L, int n_levels, int max_bins)
+HbmBlock hbm_block(int L, int n_levels, int max_bins, int sampler)
 {
     HbmBlock b;
     const size_t n_spins = (size_t)L * (size_t)L;
@@ -18,12 +18,13 @@ HbmBlock hbm_block(int L, int n_levels, int max_bins)
     b.sums = align256(b.lvl1 + n_lvl1 * 4u);
     b.state = align256(b.sums + 4u * 8u);
     b.rec = align256(b.state + 2u * 4u);
-    b.bytes = align256(b.rec + (size_t)max_bins * record_size((uint32_t)n_levels) * 8u);
+    b.labels = align256(b.rec + (size_t)max_bins * record_size((uint32_t)n_levels) * 8u);
+    b.bytes = align256(b.labels + (sampler == MCRG_SAMPLER_SWENDSEN_WANG ? n_spins * 4u : 0u));
     return b;
 }
 
 HbmLattice hbm_descriptor(char *base, const HbmBlock &b, int L, int n_levels, int sweeps_per_bin, uint64_t seed,
-                          double coupling)
+                          double coupling, int sampler)
 {
     HbmLattice h;
     std::memset(&h, 0, sizeof(h));
@@ -38,18 +39,27 @@ HbmLattice hbm_descriptor(char *base, const HbmBlock &b, int L, int n_levels, in
     h.L = L;
     h.n_levels = n_levels;
     h.sweeps_per_bin = sweeps_per_bin;
+    h.sampler = sampler;
+    h.labels = sampler == MCRG_SAMPLER_SWENDSEN_WANG ? reinterpret_cast<int32_t *>(base + b.labels) : nullptr;
+    h.bond_thr = bond_threshold(coupling);
     return h;
 }
 
-hipError_t enqueue_sweeps(const HbmLattice *d, int n_lats, int L, uint32_t n, bool measure, hipStream_t s,
-                          KernelTimer *tm)
+hipError_t enqueue_sweeps(const HbmLattice *d, int n_lats, int L, int sampler, uint32_t n, bool measure,
+                          hipStream_t s, KernelTimer *tm)
 {
     hipError_t e;
     for (uint32_t i = 0; i < n; ++i) {
-        for (uint32_t c = 0; c < 2u; ++c) {
+        if (sampler == MCRG_SAMPLER_SWENDSEN_WANG) {
             const int ev = tm ? tm->begin(kKindSweep, s) : -1;
-            if ((e = launch_int8_half_sweep(d, n_lats, L, i, c, s)) != hipSuccess) return e;
+            if ((e = launch_sw_sweep(d, n_lats, L, i, s)) != hipSuccess) return e;
             if (tm) tm->end(ev, s);
+        } else {
+            for (uint32_t c = 0; c < 2u; ++c) {
+                const int ev = tm ? tm->begin(kKindSweep, s) : -1;
+                if ((e = launch_int8_half_sweep(d, n_lats, L, i, c, s)) != hipSuccess) return e;
+                if (tm) tm->end(ev, s);
+            }
         }
         if (measure) {
             int ev = tm ? tm->begin(kKindMeasure0, s) : -1;
@@ -69,8 +79,8 @@ SweepRunner::~SweepRunner()
         if (g) (void)hipGraphExecDestroy(g);
 }
 
-hipError_t SweepRunner::run(const HbmLattice *d, int n_lats, int L, uint32_t n, bool measure, hipStream_t s,
-                            KernelTimer *tm)
+hipError_t SweepRunner::run(const HbmLattice *d, int n_lats, int L, int sampler, uint32_t n, bool measure,
+                            hipStream_t s, KernelTimer *tm)
 {
     hipError_t e;
     uint32_t done = 0;
@@ -82,7 +92,7 @@ hipError_t SweepRunner::run(const HbmLattice *d, int n_lats, int L, uint32_t n, 
             // so every replay continues where the previous one stopped
             hipGraph_t g = nullptr;
             if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
-            hipError_t ce = enqueue_sweeps(d, n_lats, L, kGraphSweeps, measure, s, nullptr);
+            hipError_t ce = enqueue_sweeps(d, n_lats, L, sampler, kGraphSweeps, measure, s, nullptr);
             if (ce == hipSuccess) ce = launch_advance(d, n_lats, kGraphSweeps, measure ? kGraphSweeps : 0u, s);
             e = hipStreamEndCapture(s, &g);
             if (ce != hipSuccess) {
@@ -102,7 +112,7 @@ hipError_t SweepRunner::run(const HbmLattice *d, int n_lats, int L, uint32_t n, 
     }
     const uint32_t rest = n - done;
     if (rest == 0) return hipSuccess;
-    if ((e = enqueue_sweeps(d, n_lats, L, rest, measure, s, tm)) != hipSuccess) return e;
+    if ((e = enqueue_sweeps(d, n_lats, L, sampler, rest, measure, s, tm)) != hipSuccess) return e;
     return launch_advance(d, n_lats, rest, measure ? rest : 0u, s);
 }
 
@@ -115,7 +125,7 @@ HbmGroup::~HbmGroup()
 bool HbmGroup::same_shape(const JobSpec &a, const JobSpec &b)
 {
     return a.L == b.L && a.n_levels == b.n_levels && a.n_therm == b.n_therm && a.n_meas == b.n_meas &&
-           a.sweeps_per_bin == b.sweeps_per_bin;
+           a.sweeps_per_bin == b.sweeps_per_bin && a.sampler == b.sampler;
 }
 
 std::string HbmGroup::check(const JobSpec &j)
@@ -127,6 +137,8 @@ std::string HbmGroup::check(const JobSpec &j)
     if (j.n_levels > 1 && j.L > kMaxHbmMeasureL) return "two or more blockings on the HBM schedule need L <= 2048";
     if (j.sweeps_per_bin < 1) return "sweeps_per_bin must be >= 1";
     if ((uint64_t)j.n_therm + (uint64_t)j.n_meas > 0xFFFFFFFFull) return "too many sweeps";
+    if (j.sampler != MCRG_SAMPLER_METROPOLIS && j.sampler != MCRG_SAMPLER_SWENDSEN_WANG)
+        return "sampler must be MCRG_SAMPLER_METROPOLIS or MCRG_SAMPLER_SWENDSEN_WANG";
     return std::string();
 }
 
@@ -138,7 +150,7 @@ hipError_t HbmGroup::init(const std::vector<JobSpec> &members, hipStream_t s)
     if (n_bins_ < 1) n_bins_ = 1;
     therm_left_ = j.n_therm;
     meas_left_ = j.n_meas;
-    blk_ = hbm_block(j.L, j.n_levels, n_bins_);
+    blk_ = hbm_block(j.L, j.n_levels, n_bins_, j.sampler);
     const size_t n = members_.size();
     const size_t desc_off = blk_.bytes * n;
     hipError_t e;
@@ -149,7 +161,7 @@ hipError_t HbmGroup::init(const std::vector<JobSpec> &members, hipStream_t s)
     host_.resize(n);
     for (size_t k = 0; k < n; ++k)
         host_[k] = hbm_descriptor(d_block_ + k * blk_.bytes, blk_, j.L, j.n_levels, j.sweeps_per_bin,
-                                  members_[k].seed, members_[k].coupling);
+                                  members_[k].seed, members_[k].coupling, j.sampler);
     if ((e = hipMemsetAsync(d_block_, 0, desc_off, s)) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(d_desc_, host_.data(), sizeof(HbmLattice) * n, hipMemcpyHostToDevice, s)) !=
         hipSuccess)
@@ -163,13 +175,13 @@ hipError_t HbmGroup::advance(uint32_t n, hipStream_t s, KernelTimer *tm)
     hipError_t e;
     const uint32_t therm = std::min(n, therm_left_);
     if (therm > 0) {
-        if ((e = runner_.run(d_desc_, size(), spec().L, therm, false, s, tm)) != hipSuccess) return e;
+        if ((e = runner_.run(d_desc_, size(), spec().L, spec().sampler, therm, false, s, tm)) != hipSuccess) return e;
         therm_left_ -= therm;
         t_ += therm;
     }
     const uint32_t meas = std::min(n - therm, meas_left_);
     if (meas > 0) {
-        if ((e = runner_.run(d_desc_, size(), spec().L, meas, true, s, tm)) != hipSuccess) return e;
+        if ((e = runner_.run(d_desc_, size(), spec().L, spec().sampler, meas, true, s, tm)) != hipSuccess) return e;
         meas_left_ -= meas;
         t_ += meas;
         q_ += meas;

@@ -16,17 +16,17 @@ namespace mcrg {
 
 // Byte offsets (256-aligned) of one lattice's buffers inside its device block.
 struct HbmBlock {
-    size_t spins = 0, lvl1 = 0, sums = 0, state = 0, rec = 0, bytes = 0;
+    size_t spins = 0, lvl1 = 0, sums = 0, state = 0, rec = 0, labels = 0, bytes = 0;
 };
-HbmBlock hbm_block(int L, int n_levels, int max_bins);
+HbmBlock hbm_block(int L, int n_levels, int max_bins, int sampler);
 // Descriptor of the lattice whose block starts at `base`.
 HbmLattice hbm_descriptor(char *base, const HbmBlock &b, int L, int n_levels, int sweeps_per_bin, uint64_t seed,
-                          double coupling);
+                          double coupling, int sampler);
 
 // Launches n sweeps (t_off 0..n-1) of n_lats lattices of side L without
 // advancing their counters.
-hipError_t enqueue_sweeps(const HbmLattice *d, int n_lats, int L, uint32_t n, bool measure, hipStream_t s,
-                          KernelTimer *tm);
+hipError_t enqueue_sweeps(const HbmLattice *d, int n_lats, int L, int sampler, uint32_t n, bool measure,
+                          hipStream_t s, KernelTimer *tm);
 
 // Runs of sweeps followed by the counter advance.  Whole blocks of
 // kGraphSweeps sweeps replay one captured graph per (descriptors, measure);
@@ -38,7 +38,8 @@ class SweepRunner {
     SweepRunner(const SweepRunner &) = delete;
     SweepRunner &operator=(const SweepRunner &) = delete;
     ~SweepRunner();
-    hipError_t run(const HbmLattice *d, int n_lats, int L, uint32_t n, bool measure, hipStream_t s, KernelTimer *tm);
+    hipError_t run(const HbmLattice *d, int n_lats, int L, int sampler, uint32_t n, bool measure, hipStream_t s,
+                   KernelTimer *tm);
 
   private:
     hipGraphExec_t exec_[2] = {nullptr, nullptr};
@@ -52,7 +53,7 @@ class HbmGroup {
     HbmGroup(const HbmGroup &) = delete;
     HbmGroup &operator=(const HbmGroup &) = delete;
     ~HbmGroup();
-    // Same-shaped jobs (L, n_levels, n_therm, n_meas, sweeps_per_bin equal).
+    // Same-shaped jobs (L, n_levels, n_therm, n_meas, sweeps_per_bin, sampler equal).
     static bool same_shape(const JobSpec &a, const JobSpec &b);
     static std::string check(const JobSpec &j);
     hipError_t init(const std::vector<JobSpec> &members, hipStream_t s);

@@ -5,6 +5,11 @@
 //
 //   k_int8_init           Philox +-1 start        (Lattice::initialize_random_spins, src/lattice.cpp:33-41)
 //   k_int8_half_sweep     checkerboard Metropolis (replaces Wolff, src/ising.cpp:87-149)
+//   k_sw_reset, k_sw_union, k_sw_flip
+//                         Swendsen-Wang cluster sweep: the Wolff construction
+//                         (src/ising.cpp:96-149) for every cluster at once, with
+//                         GPU union-find on atomicMin
+
 //   k_int8_measure0       level-0 NN/NNN/P + M and b=2 majority blocking into a
 //                         bit-packed level-1 lattice (src/lattice.cpp:102-120, src/mcrg.cpp:314-348)
 //   k_levels_accumulate   levels 1..nl in one workgroup per lattice, then the
@@ -27,6 +32,83 @@ __global__ __launch_bounds__(256) void k_int8_init(const HbmLattice *__restrict_
     const u32x4 r = philox(m >> 7, 0u, 0u, kTagInit << 28, lat.k0, lat.k1);
     const uint32_t bit = (pick4(r, (m >> 5) & 3u) >> (m & 31u)) & 1u;
     lat.spins[m] = (int8_t)(1 - 2 * (int)bit);
+}
+
+// ---------------------------------------------------------------- Swendsen-Wang
+// Cluster labels: every site starts as its own root; a union links the larger
+// root below the smaller one with atomicMin, so every tree's root is the
+// smallest site index of its cluster whatever the order of the unions.  Loads
+// of labels may return stale values while other threads unite: a label only
+// ever decreases and always names a site of the same cluster, so a stale read
+// lengthens a path but never leaves the cluster, and the atomics decide.
+__global__ __launch_bounds__(256) void k_sw_reset(const HbmLattice *__restrict__ lats)
+{
+    const HbmLattice &lat = lats[blockIdx.y];
+    const uint32_t n = (uint32_t)lat.L * (uint32_t)lat.L;
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) lat.labels[m] = (int32_t)m;
+}
+
+__device__ __forceinline__ int32_t sw_root(const int32_t *labels, int32_t i)
+{
+    for (;;) {
+        const int32_t p = __hip_atomic_load(labels + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (p == i) return i;
+        i = p;
+    }
+}
+
+__device__ __forceinline__ void sw_unite(int32_t *labels, int32_t a, int32_t b)
+{
+    for (;;) {
+        a = sw_root(labels, a);
+        b = sw_root(labels, b);
+        if (a == b) return;
+        if (a > b) {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        // link root b below a; if b stopped being a root, continue from its parent
+        const int32_t old = atomicMin(labels + b, a);
+        if (old == b) return;
+        b = old;
+    }
+}
+
+// One thread per pair of sites m = 2j, 2j+1: one Philox call gives both sites'
+// right and down bond draws.
+__global__ __launch_bounds__(256) void k_sw_union(const HbmLattice *__restrict__ lats, uint32_t t_off)
+{
+    const HbmLattice &lat = lats[blockIdx.y];
+    const uint32_t L = (uint32_t)lat.L, n = L * L;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (2u * j >= n) return;
+    const uint32_t t = lat.state[0] + t_off;
+    const u32x4 r = philox(j, t, 0u, kTagSwBond << 28, lat.k0, lat.k1);
+    const uint32_t u[4] = {r.v0, r.v1, r.v2, r.v3};
+    const uint32_t mask = L - 1u;
+#pragma unroll
+    for (uint32_t k = 0; k < 2u; ++k) {
+        const uint32_t m = 2u * j + k;
+        const uint32_t x = m & mask, y = m / L;
+        const uint32_t right = y * L + ((x + 1u) & mask), down = ((y + 1u) & mask) * L + x;
+        const int8_t sm = lat.spins[m];
+        if (sm == lat.spins[right] && u[2u * k] < lat.bond_thr) sw_unite(lat.labels, (int32_t)m, (int32_t)right);
+        if (sm == lat.spins[down] && u[2u * k + 1u] < lat.bond_thr) sw_unite(lat.labels, (int32_t)m, (int32_t)down);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sw_flip(const HbmLattice *__restrict__ lats, uint32_t t_off)
+{
+    const HbmLattice &lat = lats[blockIdx.y];
+    const uint32_t n = (uint32_t)lat.L * (uint32_t)lat.L;
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const uint32_t t = lat.state[0] + t_off;
+    const uint32_t root = (uint32_t)sw_root(lat.labels, (int32_t)m);
+    const u32x4 r = philox(root >> 7, t, 0u, kTagSwFlip << 28, lat.k0, lat.k1);
+    if ((pick4(r, (root >> 5) & 3u) >> (root & 31u)) & 1u) lat.spins[m] = (int8_t)(-lat.spins[m]);
 }
 
 // ---------------------------------------------------------------- sweep
@@ -354,6 +436,15 @@ hipError_t launch_int8_init(const HbmLattice *d, int n_lats, int L, hipStream_t 
 {
     const uint32_t n = (uint32_t)L * (uint32_t)L;
     hipLaunchKernelGGL(k_int8_init, dim3(cdiv(n, 256u), (uint32_t)n_lats), dim3(256), 0, s, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_sw_sweep(const HbmLattice *d, int n_lats, int L, uint32_t t_off, hipStream_t s)
+{
+    const uint32_t n = (uint32_t)L * (uint32_t)L;
+    hipLaunchKernelGGL(k_sw_reset, dim3(cdiv(n, 256u), (uint32_t)n_lats), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_sw_union, dim3(cdiv(cdiv(n, 2u), 256u), (uint32_t)n_lats), dim3(256), 0, s, d, t_off);
+    hipLaunchKernelGGL(k_sw_flip, dim3(cdiv(n, 256u), (uint32_t)n_lats), dim3(256), 0, s, d, t_off);
     return hipGetLastError();
 }
 

@@ -48,11 +48,16 @@ struct HbmLattice {
     int L;
     int n_levels;
     int sweeps_per_bin;
-    int pad;
+    int sampler;                // MCRG_SAMPLER_METROPOLIS or MCRG_SAMPLER_SWENDSEN_WANG
+    int32_t *labels;            // L*L cluster labels (Swendsen-Wang only)
+    uint32_t bond_thr;          // floor((1 - exp(-2J)) 2^32): bond probability of an aligned pair
+    uint32_t pad;
 };
 
 // Launchers (host).  `d` points to n_lats device descriptors that all share side L.
 hipError_t launch_int8_init(const HbmLattice *d, int n_lats, int L, hipStream_t s);
+// One Swendsen-Wang sweep: label reset, bond draw + union, root coin + flip.
+hipError_t launch_sw_sweep(const HbmLattice *d, int n_lats, int L, uint32_t t_off, hipStream_t s);
 hipError_t launch_int8_half_sweep(const HbmLattice *d, int n_lats, int L, uint32_t t_off,
                                   uint32_t colour, hipStream_t s);
 hipError_t launch_int8_measure0(const HbmLattice *d, int n_lats, int L, uint32_t t_off,

@@ -41,6 +41,8 @@ inline int ilog2(int64_t v)
 }
 
 Accept make_accept(double coupling);
+// Swendsen-Wang bond threshold floor((1 - exp(-2J)) 2^32).
+uint32_t bond_threshold(double coupling);
 
 enum KernelKind { kKindSweep = 0, kKindMeasure0 = 1, kKindLevels = 2, kKindResident = 3, kNumKinds = 4 };
 

@@ -89,7 +89,7 @@ int levels_of(int N) { return std::max(0, (int)std::floor(std::log((double)N) / 
 
 mcrg_chain new_chain(int N, double J, uint64_t seed, int n_levels, int spb, int bins)
 {
-    mcrg_chain_desc d;
+    mcrg_chain_desc d{};
     d.L = N;
     d.coupling = J;
     d.seed = seed;
@@ -97,6 +97,7 @@ mcrg_chain new_chain(int N, double J, uint64_t seed, int n_levels, int spb, int 
     d.layout = MCRG_LAYOUT_INT8;
     d.sweeps_per_bin = spb;
     d.max_bins = bins;
+    d.sampler = MCRG_SAMPLER_METROPOLIS;
     mcrg_chain ch = nullptr;
     check(mcrg_chain_create(mcrg_cpp::context(), &d, &ch));
     return ch;
@@ -321,6 +322,12 @@ void MonteCarloRenormalizationGroup::set_bins(int bins)
 void MonteCarloRenormalizationGroup::set_levels(int levels) { levels_ = levels; }
 void MonteCarloRenormalizationGroup::set_output_dir(const std::string &dir) { dir_ = dir; }
 void MonteCarloRenormalizationGroup::set_quiet(bool quiet) { quiet_ = quiet; }
+void MonteCarloRenormalizationGroup::set_sampler(int sampler)
+{
+    if (sampler != MCRG_SAMPLER_METROPOLIS && sampler != MCRG_SAMPLER_SWENDSEN_WANG)
+        throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "unknown sampler");
+    sampler_ = sampler;
+}
 
 std::vector<int64_t> MonteCarloRenormalizationGroup::run_chains(int N, double K, int n_levels, int n_samples_eq,
                                                                 int n_samples, int &n_bins)
@@ -350,6 +357,7 @@ std::vector<int64_t> MonteCarloRenormalizationGroup::run_chains(int N, double K,
             j.n_meas = n_loc[(size_t)r];
             j.n_levels = n_levels;
             j.sweeps_per_bin = spb;
+            j.sampler = sampler_;
         }
         mcrg_batch b = nullptr;
         check(mcrg_batch_create(mcrg_cpp::context(), jobs.data(), R, nullptr, &b));
@@ -388,8 +396,9 @@ double crossing(const std::vector<double> &J, const std::vector<double> &D, cons
 
 MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect_critical_point(
     int n_rounds, int n_k, double J_lo, double J_hi, int n_large, int L_large, int n_small, int L_small,
-    int n_samples_eq, int n_samples)
+    int n_samples_eq, int n_samples, int sampler_large, int n_samples_eq_small)
 {
+    if (n_samples_eq_small < 0) n_samples_eq_small = n_samples_eq;
     if (n_rounds < 1 || n_k < 2 || !(J_hi > J_lo) || !(J_lo > 0.0))
         throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "need n_rounds >= 1, n_k >= 2 and 0 < J_lo < J_hi");
     if (L_small < 8 || L_large <= L_small || (L_large % L_small) != 0 || n_large % n_k || n_small % n_k ||
@@ -416,9 +425,10 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
             const double J = lo + (hi - lo) * (q + 0.5) / n_k;
             br.J.push_back(J);
             for (int r = 0; r < per_l; ++r)
-                jobs.push_back({L_large, J, mcrg_cpp::next_seed(), n_samples_eq, n_samples, nl_large, spb});
+                jobs.push_back({L_large, J, mcrg_cpp::next_seed(), n_samples_eq, n_samples, nl_large, spb, sampler_large});
             for (int r = 0; r < per_s; ++r)
-                jobs.push_back({L_small, J, mcrg_cpp::next_seed(), n_samples_eq, n_samples, nl_small, spb});
+                jobs.push_back({L_small, J, mcrg_cpp::next_seed(), n_samples_eq_small, n_samples, nl_small, spb,
+                                MCRG_SAMPLER_METROPOLIS});
         }
         mcrg_batch b = nullptr;
         check(mcrg_batch_create(mcrg_cpp::context(), jobs.data(), (int32_t)jobs.size(), nullptr, &b));
@@ -442,7 +452,7 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
                 }
                 (large ? NL : NS)[(size_t)q * nb + bin] += (double)r[12 * (nl + 1) + 10 * nl + 3];
             }
-            out.site_updates += (double)jobs[j].L * jobs[j].L * (double)(n_samples_eq + n_samples);
+            out.site_updates += (double)jobs[j].L * jobs[j].L * (double)(jobs[j].n_therm + jobs[j].n_meas);
         }
         mcrg_batch_destroy(b);
         // D per coupling (NN per site at the matched size), all bins and each bin left out

@@ -150,6 +150,7 @@ class MonteCarloRenormalizationGroup {
     void set_levels(int levels);      // blockings; default floor(log2 N) - 1
     void set_output_dir(const std::string &dir);
     void set_quiet(bool quiet);
+    void set_sampler(int sampler);    // MCRG_SAMPLER_* for calc_critical_exponent and the locator
     // exponents of the last calc_critical_exponent, one per level
     const std::vector<mcrg_rg_result> &results() const { return results_; }
     // per-level estimates of the last approx_critical_point
@@ -173,8 +174,13 @@ class MonteCarloRenormalizationGroup {
         std::vector<BisectionRound> rounds;
         double Jc = 0, Jc_err = 0, seconds = 0, site_updates = 0;
     };
+    // The large lattices use `sampler_large` (Swendsen-Wang by default: they
+    // equilibrate in tens of sweeps where Metropolis needs ~L^2.2); the small
+    // ones run Metropolis on the LDS-resident schedule with n_samples_eq_small
+    // equilibration sweeps (-1: n_samples_eq).
     Bisection bisect_critical_point(int n_rounds, int n_k, double J_lo, double J_hi, int n_large, int L_large,
-                                    int n_small, int L_small, int n_samples_eq, int n_samples);
+                                    int n_small, int L_small, int n_samples_eq, int n_samples,
+                                    int sampler_large = MCRG_SAMPLER_SWENDSEN_WANG, int n_samples_eq_small = -1);
 
   private:
     double approx_critical_point(int n_samples_eq, int n_samples, int L, double K);
@@ -183,6 +189,7 @@ class MonteCarloRenormalizationGroup {
 
     int n_processes_ = 1;
     int n_ops_ = 2;
+    int sampler_ = MCRG_SAMPLER_METROPOLIS;
     int bins_ = 50;
     int levels_ = -1;
     int iter_ = 0;

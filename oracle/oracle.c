@@ -263,24 +263,92 @@ void orc_measure(int L, const int8_t *s, int nl, uint64_t seed, uint32_t t, int6
     rec[ORC_OFF_N(nl)] += 1;
 }
 
+/* Swendsen-Wang bond threshold floor((1 - exp(-2J)) 2^32): an aligned pair is
+ * bonded with probability 1 - exp(-2J), the cluster construction of the
+ * reference's Wolff step (P_ = 1 - exp(2K), src/ising.cpp:9,134) applied to
+ * every bond at once. */
+uint32_t orc_sw_threshold(double J)
+{
+    double v = floor(-expm1(-2.0 * J) * 4294967296.0);
+    if (v < 0.0) v = 0.0;
+    if (v > 4294967295.0) v = 4294967295.0;
+    return (uint32_t)v;
+}
+
+static int sw_find(int32_t *lab, int i)
+{
+    while (lab[i] != i) i = lab[i];
+    return i;
+}
+
+static void sw_unite(int32_t *lab, int a, int b)
+{
+    a = sw_find(lab, a);
+    b = sw_find(lab, b);
+    if (a < b) lab[b] = a;
+    else if (b < a) lab[a] = b;
+}
+
+/* One Swendsen-Wang sweep at sweep index t: the cluster move of the reference's
+ * Wolff sampler (src/ising.cpp:87-149: aligned neighbours join with probability
+ * P, the cluster flips) applied to all clusters at once.
+ *   Bonds   site m = y*L + x owns its right bond (to (x+1, y)) and its down bond
+ *           (to (x, y+1)); with out = Philox(m >> 1, t, 0, TAG_SW_BOND << 28) the
+ *           right bond draws U = out[2 (m & 1)], the down bond out[2 (m & 1) + 1];
+ *           a bond is active iff its two spins are equal and U < bthr.
+ *   Clusters the connected components of active bonds (periodic); the root of a
+ *           cluster is its smallest site index r.
+ *   Flip    every site of the cluster flips iff bit (r & 31) of out[(r >> 5) & 3]
+ *           of Philox(r >> 7, t, 0, TAG_SW_FLIP << 28) is 1. */
+void orc_sw_sweep(int L, int8_t *s, uint64_t seed, uint32_t bthr, uint32_t t)
+{
+    const int n = L * L;
+    int32_t *lab = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+    uint32_t out[4];
+    for (int m = 0; m < n; ++m) lab[m] = m;
+    for (int m = 0; m < n; ++m) {
+        const int x = m & (L - 1), y = m / L;
+        const int right = y * L + wrap(x + 1, L), down = wrap(y + 1, L) * L + x;
+        philox_seeded(seed, (uint32_t)(m >> 1), t, 0u, (uint32_t)ORC_TAG_SW_BOND << 28, out);
+        if (s[m] == s[right] && out[2 * (m & 1)] < bthr) sw_unite(lab, m, right);
+        if (s[m] == s[down] && out[2 * (m & 1) + 1] < bthr) sw_unite(lab, m, down);
+    }
+    for (int m = 0; m < n; ++m) {
+        const int r = sw_find(lab, m);
+        philox_seeded(seed, (uint32_t)(r >> 7), t, 0u, (uint32_t)ORC_TAG_SW_FLIP << 28, out);
+        if ((out[(r >> 5) & 3] >> (r & 31)) & 1u) s[m] = (int8_t)-s[m];
+    }
+    free(lab);
+}
+
 /* Thermalisation then measurement, as IsingModel::equilibrate (src/ising.cpp:39-41)
  * followed by the sampling loop of calc_critical_exponent (src/mcrg.cpp:72-98).
  * Sweep indices run t0 .. t0+n_therm+n_meas-1; sample q = t - t0 - n_therm lands in
  * bin q / sweeps_per_bin.  `s` must hold the starting configuration. */
-void orc_run_chain(int L, double J, uint64_t seed, uint32_t t0, int n_therm, int n_meas,
-                   int nl, int sweeps_per_bin, int8_t *s, int64_t *records)
+void orc_run_chain_sampler(int L, double J, uint64_t seed, uint32_t t0, int n_therm, int n_meas, int nl,
+                           int sweeps_per_bin, int sampler, int8_t *s, int64_t *records)
 {
     uint64_t thr[2];
     orc_thresholds(J, thr);
+    const uint32_t bthr = orc_sw_threshold(J);
     const int rs = ORC_RECORD_SIZE(nl);
     for (int q = 0; q < n_therm + n_meas; ++q) {
         uint32_t t = t0 + (uint32_t)q;
-        orc_sweep(L, s, seed, thr, t);
+        if (sampler == ORC_SAMPLER_SW)
+            orc_sw_sweep(L, s, seed, bthr, t);
+        else
+            orc_sweep(L, s, seed, thr, t);
         if (q >= n_therm) {
             int bin = (q - n_therm) / sweeps_per_bin;
             orc_measure(L, s, nl, seed, t, records + (size_t)bin * rs);
         }
     }
+}
+
+void orc_run_chain(int L, double J, uint64_t seed, uint32_t t0, int n_therm, int n_meas,
+                   int nl, int sweeps_per_bin, int8_t *s, int64_t *records)
+{
+    orc_run_chain_sampler(L, J, seed, t0, n_therm, n_meas, nl, sweeps_per_bin, ORC_SAMPLER_METROPOLIS, s, records);
 }
 
 /* ---------------------------------------------------------------------------

@@ -9,7 +9,8 @@
  *   reference (kim-jane/MCRG)          this file
  *   --------------------------------   ------------------------------------------
  *   src/lattice.cpp:33-41  init        orc_init_spins       (Philox instead of mt19937_64)
- *   src/ising.cpp:87-149   Wolff       orc_half_sweep/sweep (checkerboard Metropolis)
+ *   src/ising.cpp:87-149   Wolff       orc_half_sweep/sweep (checkerboard Metropolis),
+ *                                      orc_sw_sweep (Swendsen-Wang: all clusters at once)
  *   src/lattice.cpp:84-120 operators   orc_ops (single count), orc_ops_refcount
  *   src/mcrg.cpp:314-348   blocking    orc_block
  *   src/mcrg.cpp:72-98     accumulate  orc_measure
@@ -65,7 +66,8 @@
 extern "C" {
 #endif
 
-enum { ORC_TAG_SWEEP = 1, ORC_TAG_TIE = 2, ORC_TAG_INIT = 3 };
+enum { ORC_TAG_SWEEP = 1, ORC_TAG_TIE = 2, ORC_TAG_INIT = 3, ORC_TAG_SW_BOND = 4, ORC_TAG_SW_FLIP = 5 };
+enum { ORC_SAMPLER_METROPOLIS = 0, ORC_SAMPLER_SW = 1 };
 
 /* Record offsets for a chain measured on levels 0..nl (nl = number of blockings). */
 #define ORC_OFF_S(nl, l, a)        ((l) * 3 + (a))
@@ -92,6 +94,12 @@ int  orc_record_size(int nl);
 void orc_measure(int L, const int8_t *s, int nl, uint64_t seed, uint32_t t, int64_t *rec);
 void orc_run_chain(int L, double J, uint64_t seed, uint32_t t0, int n_therm, int n_meas,
                    int nl, int sweeps_per_bin, int8_t *s, int64_t *records);
+/* The same with the sweep chosen by `sampler` (ORC_SAMPLER_*). */
+void orc_run_chain_sampler(int L, double J, uint64_t seed, uint32_t t0, int n_therm, int n_meas, int nl,
+                           int sweeps_per_bin, int sampler, int8_t *s, int64_t *records);
+
+uint32_t orc_sw_threshold(double J);
+void orc_sw_sweep(int L, int8_t *s, uint64_t seed, uint32_t bthr, uint32_t t);
 
 int    orc_rg_matrix(const int64_t *rec, int nl, int level, int n_ops, double *T);
 double orc_leading_eigenvalue(int n, const double *T);

@@ -44,6 +44,10 @@ def lib():
             "orc_record_size": (c_int, [c_int]),
             "orc_measure": (None, [c_int, i8p, c_int, c_u64, c_u32, i64p]),
             "orc_run_chain": (None, [c_int, c_d, c_u64, c_u32, c_int, c_int, c_int, c_int, i8p, i64p]),
+            "orc_run_chain_sampler": (None, [c_int, c_d, c_u64, c_u32, c_int, c_int, c_int, c_int, c_int, i8p,
+                                             i64p]),
+            "orc_sw_threshold": (c_u32, [c_d]),
+            "orc_sw_sweep": (None, [c_int, i8p, c_u64, c_u32, c_u32]),
             "orc_rg_matrix": (c_int, [i64p, c_int, c_int, c_int, f64p]),
             "orc_leading_eigenvalue": (c_d, [c_int, f64p]),
             "orc_yt": (c_d, [i64p, c_int, c_int, c_int]),
@@ -111,14 +115,26 @@ def record_size(nl):
     return lib().orc_record_size(nl)
 
 
-def run_chain(L, J, seed, n_therm, n_meas, nl, sweeps_per_bin, s=None, t0=0):
+SAMPLER_METROPOLIS = 0
+SAMPLER_SW = 1
+
+
+def sw_threshold(J):
+    return lib().orc_sw_threshold(float(J))
+
+
+def sw_sweep(L, s, seed, J, t):
+    lib().orc_sw_sweep(L, s, seed, sw_threshold(J), t)
+
+
+def run_chain(L, J, seed, n_therm, n_meas, nl, sweeps_per_bin, s=None, t0=0, sampler=SAMPLER_METROPOLIS):
     """Returns (final spins, records[n_bins, record_size])."""
     if s is None:
         s = init_spins(L, seed)
     s = np.ascontiguousarray(s, dtype=np.int8).copy()
     n_bins = max(1, -(-n_meas // sweeps_per_bin)) if n_meas > 0 else 1
     rec = np.zeros(n_bins * record_size(nl), dtype=np.int64)
-    lib().orc_run_chain(L, float(J), seed, t0, n_therm, n_meas, nl, sweeps_per_bin, s, rec)
+    lib().orc_run_chain_sampler(L, float(J), seed, t0, n_therm, n_meas, nl, sweeps_per_bin, sampler, s, rec)
     return s, rec.reshape(n_bins, record_size(nl))
 
 

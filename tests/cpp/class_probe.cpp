@@ -2,6 +2,7 @@
 // with the oracle: the chain seed, spins after a known sequence of calls, and
 // the operator values the classes report.
 #include <cstdio>
+#include <cstring>
 #include <memory>
 
 #include "../../mcrg_amd/host/mcrg.hpp"
@@ -27,6 +28,13 @@ int main(int argc, char **argv)
     for (int k = 0; k < 4; ++k) std::printf("%s[%d, %d]", k ? ", " : "", nn(k, 0), nn(k, 1));
     std::printf("], \"nnn\": [");
     for (int k = 0; k < 4; ++k) std::printf("%s[%d, %d]", k ? ", " : "", nnn(k, 0), nnn(k, 1));
-    std::printf("]}\n");
+    std::printf("]");
+    if (argc > 3 && std::strcmp(argv[3], "eq") == 0) {
+        // equilibration log of a second chain (file written to the working directory)
+        auto lat2 = std::make_shared<Lattice>(N);
+        model.equilibrate(lat2, 100, true);
+        std::printf(", \"seed2\": %llu", (unsigned long long)lat2->seed());
+    }
+    std::printf("}\n");
     return 0;
 }

@@ -148,3 +148,20 @@ def test_hbm_groups_bit_exact(amd, ctx, orc, chunk):
         b.run_all()
         b.sync()
         _check_vs_oracle(orc, b, jobs)
+
+
+def test_swendsen_wang_jobs_in_batch(amd, ctx, orc):
+    """Swendsen-Wang jobs run on the HBM schedule beside Metropolis jobs of the
+    same L on the resident schedule; both equal the oracle."""
+    jobs = [dict(L=64, coupling=KC, seed=700 + i, n_therm=4, n_meas=36, n_levels=3, sweeps_per_bin=12,
+                 sampler=amd.SAMPLER_SWENDSEN_WANG if i % 2 else amd.SAMPLER_METROPOLIS) for i in range(6)]
+    jobs.append(dict(L=16, coupling=0.6, seed=799, n_therm=0, n_meas=40, n_levels=2, sweeps_per_bin=40,
+                     sampler=amd.SAMPLER_SWENDSEN_WANG))
+    with amd.Batch(ctx, jobs, sweeps_per_launch=250) as b:
+        b.run_all()
+        b.sync()
+        for j, job in enumerate(jobs):
+            s_ref, r_ref = orc.run_chain(job["L"], job["coupling"], job["seed"], job["n_therm"], job["n_meas"],
+                                         job["n_levels"], job["sweeps_per_bin"], sampler=job["sampler"])
+            assert np.array_equal(b.spins(j), s_ref), job
+            assert np.array_equal(b.records(j), r_ref), job

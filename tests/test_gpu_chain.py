@@ -190,3 +190,24 @@ def test_graph_replays_match_oracle(amd, ctx, orc):
             assert np.array_equal(ch.spins(), s), (n, meas, J)
             assert ch.state() == (t, q)
             assert np.array_equal(ch.records(), rec_ref), (n, meas, J)
+
+
+@pytest.mark.parametrize("L,J,n_therm,n_meas,nl", [(2, KC, 3, 5, 0), (4, 0.3, 2, 6, 1), (8, KC, 1, 40, 2),
+                                                   (32, KC, 5, 9, 3), (64, 0.5, 33, 3, 4), (256, KC, 2, 4, 5),
+                                                   (1024, KC, 1, 2, 4)])
+def test_swendsen_wang_bit_exact(amd, ctx, orc, L, J, n_therm, n_meas, nl):
+    """Swendsen-Wang chains: spins and records equal the oracle's union-find
+    restatement (cluster roots are the smallest site index, so the partition
+    and every flip are schedule-independent)."""
+    seed = 0x5EED0000 + L
+    _say("swendsen-wang", L, J)
+    assert amd.bond_threshold(J) == orc.sw_threshold(J)
+    spb = max(1, n_meas // 2)
+    nb = amd.n_bins(n_meas, spb)
+    with amd.Chain(ctx, L, J, seed, n_levels=nl, sweeps_per_bin=spb, max_bins=nb,
+                   sampler=amd.SAMPLER_SWENDSEN_WANG) as ch:
+        ch.sweep(n_therm)
+        ch.sweep(n_meas, measure=True)
+        s_ref, r_ref = orc.run_chain(L, J, seed, n_therm, n_meas, nl, spb, sampler=orc.SAMPLER_SW)
+        assert np.array_equal(ch.spins(), s_ref)
+        assert np.array_equal(ch.records(), r_ref)

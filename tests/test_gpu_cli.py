@@ -129,3 +129,30 @@ def test_bisection_driver(tmp_path):
     assert r["Jc"] == jc2 and r["Jc_err"] == e2
     assert abs(r["Jc"] - r["exact"]) < 0.01
     assert r["site_updates"] == 2 * (16 * 64 * 64 + 64 * 16 * 16) * 4500
+
+
+def test_equilibrate_log_matches_oracle(orc, tmp_path):
+    """IsingModel::equilibrate(.., write = true): the reference's file name,
+    header and write_iter cadence (src/ising.cpp:14-84,
+    src/definitions.cpp:44-68); E/spin and |M|/spin of the chain after n
+    sweeps equal the oracle's."""
+    N, K = 16, -0.4406868
+    got = json.loads(_run([PROBE, str(N), repr(K), "eq"], tmp_path))
+    lines = open(tmp_path / "equilibrate_N_16_K_-0.4406868.txt").read().splitlines()
+    assert lines[0] == "# Nearest neighbor coupling K = -0.440687"
+    assert lines[2] == f"# Number of lattice sites = {N * N}"
+    assert lines[5].startswith("# Iteration, Avg E/spin")
+    rows = [l for l in lines[6:] if l.count(",") == 6]
+    iters = [int(r.split(",")[0]) for r in rows]
+    assert iters == list(range(1, 10)) + list(range(10, 101, 10))
+    seed = got["seed2"]
+    s = orc.init_spins(N, seed)
+    t = 0
+    for r in rows:
+        n, E, _, _, M = [float(x) for x in r.split(",")[:5]]
+        while t < n:
+            orc.sweep(N, s, seed, -K, t)
+            t += 1
+        nn = orc.ops(N, s)[0]
+        assert E == pytest.approx(K * 2 * nn / N**2, rel=1e-6, abs=1e-12)
+        assert M == pytest.approx(abs(int(s.astype(int).sum())) / N**2, rel=1e-6, abs=1e-12)

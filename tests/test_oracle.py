@@ -181,15 +181,16 @@ def _exact_T(ex, n_ops):
     return np.linalg.solve(A[:n_ops, :n_ops], B[:n_ops, :n_ops])
 
 
-@pytest.mark.parametrize("J", [KC, 0.3])
-def test_metropolis_samples_exact_4x4_ensemble(orc, J):
-    """The sweep's stationary distribution is Boltzmann: MC means of every
-    operator and of M^2 agree with exact enumeration of all 2^16 states."""
+@pytest.mark.parametrize("J,sampler", [(KC, 0), (0.3, 0), (KC, 1), (0.6, 1)])
+def test_sampler_samples_exact_4x4_ensemble(orc, J, sampler):
+    """The sweep's stationary distribution is Boltzmann (Metropolis = 0,
+    Swendsen-Wang = 1): MC means of every operator and of M^2 agree with exact
+    enumeration of all 2^16 states."""
     ex = _exact_L4(J)
     nl = 1
     bin_len = 500
     _, rec = orc.run_chain(4, J, seed=2024, n_therm=200, n_meas=100000, nl=nl,
-                           sweeps_per_bin=bin_len)
+                           sweeps_per_bin=bin_len, sampler=sampler)
     n = rec[:, orc.off_N(nl)].astype(float)
     for a in range(3):
         means = rec[:, orc.off_S(nl, 0, a)] / n
@@ -343,3 +344,44 @@ def kc_known_answer_records(orc):
 def test_kc_step_known_answer(orc):
     rl, rs = kc_known_answer_records(orc)
     assert orc.kc_step(rl, 2, rs, 1, 0) == pytest.approx(1.0 / 6.0, rel=1e-12)
+
+
+# ------------------------------------------------------------------ Swendsen-Wang
+def _coin(orc, seed, r, t, tag=5):
+    out = orc.philox(seed, [r >> 7, t, 0, tag << 28])
+    return (int(out[(r >> 5) & 3]) >> (r & 31)) & 1
+
+
+def test_sw_threshold_closed_form(orc):
+    # exp(-2 K_c) = sqrt(2) - 1, so the bond probability is 2 - sqrt(2)
+    assert orc.sw_threshold(KC) == math.floor((2.0 - math.sqrt(2.0)) * 2**32)
+    assert orc.sw_threshold(0.0) == 0
+    assert orc.sw_threshold(50.0) == 2**32 - 1
+
+
+def test_sw_singletons_at_zero_coupling(orc):
+    """J = 0: no bonds, every site is its own cluster and flips with its own coin."""
+    L, seed, t = 8, 77, 5
+    s0 = orc.init_spins(L, seed)
+    s = s0.copy()
+    orc.sw_sweep(L, s, seed, 0.0, t)
+    want = np.array([-v if _coin(orc, seed, m, t) else v for m, v in enumerate(s0)], dtype=np.int8)
+    assert np.array_equal(s, want)
+
+
+def test_sw_clusters_and_roots(orc):
+    """J large: every aligned pair bonds, so clusters are the aligned domains.
+    All-up is one cluster with root 0; vertical stripes of width 2 are L/2
+    clusters whose roots are the sites (2k, 0)."""
+    L, seed, t = 8, 91, 3
+    up = np.ones(L * L, dtype=np.int8)
+    s = up.copy()
+    orc.sw_sweep(L, s, seed, 50.0, t)
+    assert np.array_equal(s, up * (-1 if _coin(orc, seed, 0, t) else 1))
+    y, x = np.divmod(np.arange(L * L), L)
+    stripes = np.where((x // 2) % 2 == 0, 1, -1).astype(np.int8)
+    s = stripes.copy()
+    orc.sw_sweep(L, s, seed, 50.0, t)
+    root = 2 * (x // 2)
+    want = np.array([-v if _coin(orc, seed, int(r), t) else v for v, r in zip(stripes, root)], dtype=np.int8)
+    assert np.array_equal(s, want)
