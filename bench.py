@@ -21,11 +21,13 @@ Measured beside it (same JSON line):
                manifest (threads over independent replicas)
   bisection    the critical-coupling bisection (mcrg_run --bisect): per round 16
                couplings spanning the bracket (start [0.43, 0.45]), 64 lattices
-               of L = 1024 (int8, HBM schedule) and 1024 of L = 128 (bit-packed,
-               LDS-resident) in one batch, 2000 + 10000 sweeps each, NN per site
-               of L = 1024 blocked 4 times against L = 128 blocked once; 8 rounds;
-               reports the final K_c with its jackknife error against the exact
-               0.4406868 and the aggregate updates/ns
+               of L = 1024 (int8, HBM schedule, Swendsen-Wang, 1000 + 10000
+               sweeps) and 1024 of L = 128 (bit-packed, LDS-resident, Metropolis,
+               50000 + 10000 sweeps) in one batch; NN per site of L = 1024
+               blocked 3 + m times against L = 128 blocked m times (m = 0..3);
+               8 rounds; reports the final K_c with its jackknife error and the
+               spread over matched levels against the exact 0.4406868, and the
+               aggregate updates/ns
   serving      open-loop Poisson job streams (mcrg_amd.workloads.serving_stream:
                L = 2^k, k in 4..9, |K| ~ U[0.42, 0.47], 1e3 thermalisation plus
                U[1e3, 2e4] measured sweeps, blocking to L = 4, seed = job id)
@@ -212,15 +214,16 @@ def cpu_baseline(jobs, sweeps, threads):
 
 def run_bisection(rounds):
     import subprocess
-    cmd = [os.path.join(ROOT, "mcrg_amd", "mcrg_run"), "--bisect", str(rounds), "--eq", "2000", "--samples",
-           "10000", "--bins", "20", "--quiet"]
+    cmd = [os.path.join(ROOT, "mcrg_amd", "mcrg_run"), "--bisect", str(rounds), "--eq", "1000", "--eq-small",
+           "50000", "--samples", "10000", "--bins", "20", "--quiet"]
     log("bench: " + " ".join(cmd[1:]))
     out = subprocess.run(cmd, capture_output=True, text=True, timeout=1200)
     if out.returncode != 0:
         raise RuntimeError(f"mcrg_run --bisect failed: {out.stderr.strip()}")
     res = json.loads(out.stdout.strip().splitlines()[-1])
-    res["config"] = ("64 x L=1024 int8 + 1024 x L=128 packed per round, 16 couplings, 2000 + 10000 sweeps, "
-                     "NN per site of L=1024 blocked 4x vs L=128 blocked 1x")
+    res["config"] = ("per round 16 couplings; 64 x L=1024 int8 Swendsen-Wang (1000 + 10000 sweeps) and "
+                     "1024 x L=128 bit-packed Metropolis (50000 + 10000 sweeps) in one batch; NN per site of "
+                     "L=1024 blocked 3+m times vs L=128 blocked m times, m = 0..3, crossing at m = 3")
     return res
 
 
@@ -274,7 +277,8 @@ def main():
     bisection = None
     if args.bisect_rounds and not args.only:
         bisection = run_bisection(args.bisect_rounds)
-        log(f"bench: bisection Kc = {bisection['Jc']:.7f} +- {bisection['Jc_err']:.7f} "
+        log(f"bench: bisection Kc = {bisection['Jc']:.7f} +- {bisection['Jc_err']:.7f} (stat) "
+            f"+- {bisection['Jc_sys']:.7f} (levels) "
             f"({bisection['updates_per_ns']:.1f} updates/ns, {bisection['seconds']:.1f} s)")
 
     import mcrg_amd

@@ -5,7 +5,7 @@
 //
 //   k_int8_init           Philox +-1 start        (Lattice::initialize_random_spins, src/lattice.cpp:33-41)
 //   k_int8_half_sweep     checkerboard Metropolis (replaces Wolff, src/ising.cpp:87-149)
-//   k_sw_reset, k_sw_union, k_sw_flip
+//   k_sw_local, k_sw_boundary, k_sw_flip
 //                         Swendsen-Wang cluster sweep: the Wolff construction
 //                         (src/ising.cpp:96-149) for every cluster at once, with
 //                         GPU union-find on atomicMin
@@ -35,19 +35,20 @@ __global__ __launch_bounds__(256) void k_int8_init(const HbmLattice *__restrict_
 }
 
 // ---------------------------------------------------------------- Swendsen-Wang
-// Cluster labels: every site starts as its own root; a union links the larger
-// root below the smaller one with atomicMin, so every tree's root is the
-// smallest site index of its cluster whatever the order of the unions.  Loads
-// of labels may return stale values while other threads unite: a label only
-// ever decreases and always names a site of the same cluster, so a stale read
-// lengthens a path but never leaves the cluster, and the atomics decide.
-__global__ __launch_bounds__(256) void k_sw_reset(const HbmLattice *__restrict__ lats)
-{
-    const HbmLattice &lat = lats[blockIdx.y];
-    const uint32_t n = (uint32_t)lat.L * (uint32_t)lat.L;
-    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < n) lat.labels[m] = (int32_t)m;
-}
+// Cluster labels form a union-find forest in which every site points to a
+// smaller (or equal) site index, so each tree's root is the smallest site of
+// its cluster whatever the order of the unions.  Two passes build it:
+//   k_sw_local     one workgroup per tile of up to 32 x 32 sites: bond draws,
+//                  union-find of the bonds inside the tile in LDS, then every
+//                  site's global label = its tile-local root (tile rows are
+//                  row-major segments, so the local minimum is the global one)
+//   k_sw_boundary  bonds that leave a tile (right edge, bottom edge), united
+//                  in the global labels with atomicMin
+// Loads of global labels may return stale values while other threads unite:
+// a label only decreases and always names a site of the same cluster, so a
+// stale read lengthens a path but never leaves the cluster, and the atomics
+// decide.
+constexpr uint32_t kSwTile = 32;
 
 __device__ __forceinline__ int32_t sw_root(const int32_t *labels, int32_t i)
 {
@@ -76,27 +77,94 @@ __device__ __forceinline__ void sw_unite(int32_t *labels, int32_t a, int32_t b)
     }
 }
 
-// One thread per pair of sites m = 2j, 2j+1: one Philox call gives both sites'
-// right and down bond draws.
-__global__ __launch_bounds__(256) void k_sw_union(const HbmLattice *__restrict__ lats, uint32_t t_off)
+__device__ __forceinline__ int32_t lds_root(const int32_t *lab, int32_t i)
+{
+    for (;;) {
+        const int32_t p = lab[i];
+        if (p == i) return i;
+        i = p;
+    }
+}
+
+__device__ __forceinline__ void lds_unite(int32_t *lab, int32_t a, int32_t b)
+{
+    for (;;) {
+        a = lds_root(lab, a);
+        b = lds_root(lab, b);
+        if (a == b) return;
+        if (a > b) {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        const int32_t old = atomicMin(lab + b, a);
+        if (old == b) return;
+        b = old;
+    }
+}
+
+// Tile of TW x TH sites, TW = TH = min(L, 32); blockIdx.x = tile (row-major).
+// Thread pairs of horizontally adjacent sites (2p, 2p+1) share one Philox call,
+// which holds both sites' right and down bond draws.
+__global__ __launch_bounds__(256) void k_sw_local(const HbmLattice *__restrict__ lats, uint32_t t_off)
+{
+    __shared__ int32_t lab[kSwTile * kSwTile];
+    const HbmLattice &lat = lats[blockIdx.y];
+    const uint32_t L = (uint32_t)lat.L, mask = L - 1u;
+    const uint32_t TW = L < kSwTile ? L : kSwTile, nt = TW * TW;
+    const uint32_t tiles_x = L / TW;
+    const uint32_t x0 = (blockIdx.x % tiles_x) * TW, y0 = (blockIdx.x / tiles_x) * TW;
+    const bool wrap = TW == L;  // one tile holds the whole lattice: periodic bonds stay inside
+    const uint32_t t = lat.state[0] + t_off;
+    for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) lab[i] = (int32_t)i;
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; 2u * p < nt; p += blockDim.x) {
+        const uint32_t i0 = 2u * p, ly = i0 / TW, lx0 = i0 % TW;
+        const uint32_t m0 = (y0 + ly) * L + x0 + lx0;
+        const u32x4 r = philox(m0 >> 1, t, 0u, kTagSwBond << 28, lat.k0, lat.k1);
+        const uint32_t u[4] = {r.v0, r.v1, r.v2, r.v3};
+#pragma unroll
+        for (uint32_t k = 0; k < 2u; ++k) {
+            const uint32_t lx = lx0 + k, m = m0 + k;
+            const int8_t sm = lat.spins[m];
+            if (lx + 1u < TW || wrap) {
+                const uint32_t nb = (y0 + ly) * L + ((x0 + lx + 1u) & mask);
+                if (sm == lat.spins[nb] && u[2u * k] < lat.bond_thr)
+                    lds_unite(lab, (int32_t)(ly * TW + lx), (int32_t)(ly * TW + ((lx + 1u) & (TW - 1u))));
+            }
+            if (ly + 1u < TW || wrap) {
+                const uint32_t nb = ((y0 + ly + 1u) & mask) * L + x0 + lx;
+                if (sm == lat.spins[nb] && u[2u * k + 1u] < lat.bond_thr)
+                    lds_unite(lab, (int32_t)(ly * TW + lx), (int32_t)(((ly + 1u) & (TW - 1u)) * TW + lx));
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nt; i += blockDim.x) {
+        const uint32_t rt = (uint32_t)lds_root(lab, (int32_t)i);
+        lat.labels[(y0 + i / TW) * L + x0 + i % TW] = (int32_t)((y0 + rt / TW) * L + x0 + rt % TW);
+    }
+}
+
+// Bonds leaving a tile: thread b < L^2/32 takes the right bond of a site on a
+// tile's right edge, the rest the down bond of a site on a bottom edge.
+__global__ __launch_bounds__(256) void k_sw_boundary(const HbmLattice *__restrict__ lats, uint32_t t_off)
 {
     const HbmLattice &lat = lats[blockIdx.y];
-    const uint32_t L = (uint32_t)lat.L, n = L * L;
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (2u * j >= n) return;
+    const uint32_t L = (uint32_t)lat.L, mask = L - 1u;
+    const uint32_t half = L * (L / kSwTile);
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= 2u * half) return;
+    const bool down = b >= half;
+    const uint32_t c = down ? b - half : b;
+    const uint32_t edge = (c / L) * kSwTile + kSwTile - 1u, along = c % L;
+    const uint32_t x = down ? along : edge, y = down ? edge : along;
+    const uint32_t m = y * L + x;
+    const uint32_t nb = down ? ((y + 1u) & mask) * L + x : y * L + ((x + 1u) & mask);
     const uint32_t t = lat.state[0] + t_off;
-    const u32x4 r = philox(j, t, 0u, kTagSwBond << 28, lat.k0, lat.k1);
-    const uint32_t u[4] = {r.v0, r.v1, r.v2, r.v3};
-    const uint32_t mask = L - 1u;
-#pragma unroll
-    for (uint32_t k = 0; k < 2u; ++k) {
-        const uint32_t m = 2u * j + k;
-        const uint32_t x = m & mask, y = m / L;
-        const uint32_t right = y * L + ((x + 1u) & mask), down = ((y + 1u) & mask) * L + x;
-        const int8_t sm = lat.spins[m];
-        if (sm == lat.spins[right] && u[2u * k] < lat.bond_thr) sw_unite(lat.labels, (int32_t)m, (int32_t)right);
-        if (sm == lat.spins[down] && u[2u * k + 1u] < lat.bond_thr) sw_unite(lat.labels, (int32_t)m, (int32_t)down);
-    }
+    const u32x4 r = philox(m >> 1, t, 0u, kTagSwBond << 28, lat.k0, lat.k1);
+    const uint32_t u = pick4(r, 2u * (m & 1u) + (down ? 1u : 0u));
+    if (lat.spins[m] == lat.spins[nb] && u < lat.bond_thr) sw_unite(lat.labels, (int32_t)m, (int32_t)nb);
 }
 
 __global__ __launch_bounds__(256) void k_sw_flip(const HbmLattice *__restrict__ lats, uint32_t t_off)
@@ -442,8 +510,12 @@ hipError_t launch_int8_init(const HbmLattice *d, int n_lats, int L, hipStream_t 
 hipError_t launch_sw_sweep(const HbmLattice *d, int n_lats, int L, uint32_t t_off, hipStream_t s)
 {
     const uint32_t n = (uint32_t)L * (uint32_t)L;
-    hipLaunchKernelGGL(k_sw_reset, dim3(cdiv(n, 256u), (uint32_t)n_lats), dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_sw_union, dim3(cdiv(cdiv(n, 2u), 256u), (uint32_t)n_lats), dim3(256), 0, s, d, t_off);
+    const uint32_t tw = (uint32_t)L < kSwTile ? (uint32_t)L : kSwTile;
+    hipLaunchKernelGGL(k_sw_local, dim3(n / (tw * tw), (uint32_t)n_lats), dim3(256), 0, s, d, t_off);
+    if ((uint32_t)L > kSwTile) {
+        const uint32_t nb = 2u * (uint32_t)L * ((uint32_t)L / kSwTile);
+        hipLaunchKernelGGL(k_sw_boundary, dim3(cdiv(nb, 256u), (uint32_t)n_lats), dim3(256), 0, s, d, t_off);
+    }
     hipLaunchKernelGGL(k_sw_flip, dim3(cdiv(n, 256u), (uint32_t)n_lats), dim3(256), 0, s, d, t_off);
     return hipGetLastError();
 }

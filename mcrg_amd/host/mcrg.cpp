@@ -408,7 +408,12 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
     int k = 0;
     while ((L_small << k) < L_large) ++k;
     if ((L_small << k) != L_large) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "L_large / L_small must be 2^k");
-    const int nl_small = 1, nl_large = k + nl_small;  // matched at m = 0 (L_small) and m = 1 (L_small / 2)
+    // matched levels m = 0 .. n_match-1 (sizes L_small / 2^m, down to 16 sites a side)
+    int n_match = 1;
+    while (n_match < 4 && (L_small >> n_match) >= 16) ++n_match;
+    const int nl_small = n_match - 1, nl_large = k + nl_small, deep = n_match - 1;
+    if (nl_large > 1 && L_large > 2048)
+        throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "L_large above 2048 allows one blocking only");
     const int spb = std::max(1, n_samples / std::max(2, bins_));
     const int nb = (n_samples + spb - 1) / spb;
     const int rs_l = mcrg_record_size(nl_large), rs_s = mcrg_record_size(nl_small);
@@ -435,7 +440,7 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
         int32_t left = 1;
         while (left > 0) check(mcrg_batch_run(b, 1 << 30, &left));
         // per coupling and bin: NN sums at the matched levels and sample counts
-        std::vector<double> SL((size_t)n_k * nb * 2, 0.0), SS((size_t)n_k * nb * 2, 0.0);
+        std::vector<double> SL((size_t)n_k * nb * n_match, 0.0), SS((size_t)n_k * nb * n_match, 0.0);
         std::vector<double> NL((size_t)n_k * nb, 0.0), NS((size_t)n_k * nb, 0.0);
         std::vector<int64_t> rec;
         for (size_t j = 0; j < jobs.size(); ++j) {
@@ -446,9 +451,9 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
             check(mcrg_batch_get_records(b, (int32_t)j, rec.data(), (int64_t)rec.size()));
             for (int bin = 0; bin < nb; ++bin) {
                 const int64_t *r = rec.data() + (size_t)bin * rs;
-                for (int m = 0; m < 2; ++m) {
+                for (int m = 0; m < n_match; ++m) {
                     const int lvl = large ? k + m : m;
-                    (large ? SL : SS)[((size_t)q * nb + bin) * 2 + m] += (double)r[lvl * 3];
+                    (large ? SL : SS)[((size_t)q * nb + bin) * n_match + m] += (double)r[lvl * 3];
                 }
                 (large ? NL : NS)[(size_t)q * nb + bin] += (double)r[12 * (nl + 1) + 10 * nl + 3];
             }
@@ -460,8 +465,8 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
             double sl = 0, ss = 0, nl_ = 0, ns_ = 0;
             for (int bin = 0; bin < nb; ++bin) {
                 if (bin == skip) continue;
-                sl += SL[((size_t)q * nb + bin) * 2 + m];
-                ss += SS[((size_t)q * nb + bin) * 2 + m];
+                sl += SL[((size_t)q * nb + bin) * n_match + m];
+                ss += SS[((size_t)q * nb + bin) * n_match + m];
                 nl_ += NL[(size_t)q * nb + bin];
                 ns_ += NS[(size_t)q * nb + bin];
             }
@@ -471,26 +476,32 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
         const double jk = (double)(nb - 1) / nb;
         std::vector<double> w((size_t)n_k);
         for (int q = 0; q < n_k; ++q) {
-            const double d = D_of(q, 1, -1);
+            const double d = D_of(q, deep, -1);
             double var = 0.0;
-            for (int bin = 0; bin < nb; ++bin) var += (D_of(q, 1, bin) - d) * (D_of(q, 1, bin) - d);
+            for (int bin = 0; bin < nb; ++bin) var += (D_of(q, deep, bin) - d) * (D_of(q, deep, bin) - d);
             var *= jk;
             br.D.push_back(d);
             br.D_err.push_back(std::sqrt(var));
-            br.D0.push_back(D_of(q, 0, -1));
+            for (int m = 0; m < n_match; ++m) br.D_level.push_back(D_of(q, m, -1));
             w[(size_t)q] = var > 0.0 ? 1.0 / var : 0.0;
         }
         br.Jc = crossing(br.J, br.D, w);
+        for (int m = 0; m < n_match; ++m) {
+            std::vector<double> Dm((size_t)n_k);
+            for (int q = 0; q < n_k; ++q) Dm[(size_t)q] = br.D_level[(size_t)q * n_match + m];
+            br.Jc_level.push_back(crossing(br.J, Dm, w));
+        }
         double var = 0.0;
         std::vector<double> Dq((size_t)n_k);
         for (int bin = 0; bin < nb; ++bin) {
-            for (int q = 0; q < n_k; ++q) Dq[(size_t)q] = D_of(q, 1, bin);
+            for (int q = 0; q < n_k; ++q) Dq[(size_t)q] = D_of(q, deep, bin);
             const double c = crossing(br.J, Dq, w);
             var += (c - br.Jc) * (c - br.Jc);
         }
         br.Jc_err = std::sqrt(jk * var);
-        if (!quiet_)
-            std::printf("round %i: J in [%.7f, %.7f] -> Jc = %.7f +- %.7f\n", round + 1, lo, hi, br.Jc, br.Jc_err);
+        // progress on stderr, also when quiet
+        std::fprintf(stderr, "round %i: J in [%.7f, %.7f] -> Jc = %.7f +- %.7f\n", round + 1, lo, hi, br.Jc,
+                     br.Jc_err);
         out.rounds.push_back(br);
         if (!std::isfinite(br.Jc)) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "no crossing: D(J) has no slope");
         const double half = std::max(3.0 * br.Jc_err, (hi - lo) / 16.0);

@@ -162,13 +162,17 @@ class MonteCarloRenormalizationGroup {
     // couplings J spanning [J_lo, J_hi], n_large / n_k lattices of L_large and
     // n_small / n_k of L_small in one batch, and compares the nearest-neighbour
     // correlation per site of L_large blocked k + m times with L_small blocked
-    // m times (k = log2(L_large / L_small), m = 0, 1: same final size).  D(J) =
-    // large - small changes sign at the critical coupling; a weighted straight
-    // line through D(J) gives the crossing J0 and its delete-one-bin jackknife
-    // error, and the next bracket is J0 +- max(3 err, width / 8).  J = |K|.
+    // m times (k = log2(L_large / L_small); matched levels m = 0..3 while
+    // L_small / 2^m >= 16: same final size).  D(J) = large - small at the
+    // deepest matched level changes sign at the critical coupling; a weighted
+    // straight line through D(J) gives the crossing J0 and its delete-one-bin
+    // jackknife error, and the next bracket is J0 +- max(3 err, width / 16).
+    // J = |K|.
     struct BisectionRound {
         double J_lo = 0, J_hi = 0, Jc = 0, Jc_err = 0;
-        std::vector<double> J, D, D_err, D0;
+        std::vector<double> J, D, D_err;
+        std::vector<double> D_level;   // [coupling][matched level m]
+        std::vector<double> Jc_level;  // crossing at each matched level
     };
     struct Bisection {
         std::vector<BisectionRound> rounds;

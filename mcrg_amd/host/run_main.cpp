@@ -3,6 +3,7 @@
 // 1e4 equilibration and 1e6 measured samples, calc_critical_exponent at
 // K = -0.44 and at K_c = -ln(1 + sqrt 2)/2.  Flags change the sizes and add the
 // critical-point locator (src/mcrg.cpp:146-188).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -100,14 +101,26 @@ int main(int argc, char **argv)
             const auto r = MCRG.bisect_critical_point(bisect, n_k, J_lo, J_hi, n_large, L_large, n_small, L_small,
                                                       n_eq, n, sampler_large, n_eq_small);
             const double exact = std::log(1.0 + std::sqrt(2.0)) / 2.0;
-            std::printf("{\"Jc\": %.10f, \"Jc_err\": %.10f, \"exact\": %.10f, \"deviation_sigma\": %.3f, "
+            // systematic: spread of the crossings over the matched levels of the last round
+            const auto &lv0 = r.rounds.back().Jc_level;
+            double lo_c = r.Jc, hi_c = r.Jc;
+            for (double c : lv0) {
+                lo_c = std::min(lo_c, c);
+                hi_c = std::max(hi_c, c);
+            }
+            const double sys = 0.5 * (hi_c - lo_c), tot = std::sqrt(r.Jc_err * r.Jc_err + sys * sys);
+            std::printf("{\"Jc\": %.10f, \"Jc_err\": %.10f, \"Jc_sys\": %.10f, \"exact\": %.10f, "
+                        "\"deviation\": %.10f, \"deviation_sigma\": %.3f, "
                         "\"rounds\": %i, \"seconds\": %.3f, \"site_updates\": %.6e, \"updates_per_ns\": %.3f, "
                         "\"round_Jc\": [",
-                        r.Jc, r.Jc_err, exact, r.Jc_err > 0 ? (r.Jc - exact) / r.Jc_err : 0.0, (int)r.rounds.size(),
-                        r.seconds, r.site_updates, r.site_updates / (r.seconds * 1e9));
+                        r.Jc, r.Jc_err, sys, exact, r.Jc - exact, tot > 0 ? (r.Jc - exact) / tot : 0.0,
+                        (int)r.rounds.size(), r.seconds, r.site_updates, r.site_updates / (r.seconds * 1e9));
             for (size_t i = 0; i < r.rounds.size(); ++i)
                 std::printf("%s[%.10f, %.10f, %.10f, %.10f]", i ? ", " : "", r.rounds[i].J_lo, r.rounds[i].J_hi,
                             r.rounds[i].Jc, r.rounds[i].Jc_err);
+            std::printf("], \"last_round_Jc_by_level\": [");
+            const auto &lv = r.rounds.back().Jc_level;
+            for (size_t i = 0; i < lv.size(); ++i) std::printf("%s%.10f", i ? ", " : "", lv[i]);
             std::printf("]}\n");
         } else if (locate > 0) {
             MCRG.locate_critical_point(locate, n_eq, n, Lloc, K0);
