@@ -25,9 +25,10 @@ Measured beside it (same JSON line):
                sweeps) and 1024 of L = 128 (bit-packed, LDS-resident, Metropolis,
                50000 + 10000 sweeps) in one batch; NN per site of L = 1024
                blocked 3 + m times against L = 128 blocked m times (m = 0..3);
-               8 rounds; reports the final K_c with its jackknife error and the
-               spread over matched levels against the exact 0.4406868, and the
-               aggregate updates/ns
+               8 rounds; reports the final K_c with its jackknife error (groups
+               of independent chains) and the change between the two deepest
+               matched levels, against the exact 0.4406868, and the aggregate
+               updates/ns
   serving      open-loop Poisson job streams (mcrg_amd.workloads.serving_stream:
                L = 2^k, k in 4..9, |K| ~ U[0.42, 0.47], 1e3 thermalisation plus
                U[1e3, 2e4] measured sweeps, blocking to L = 4, seed = job id)
@@ -250,7 +251,7 @@ def main():
     ap.add_argument("--threads", type=int, default=256)
     ap.add_argument("--replicas", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=50)
-    ap.add_argument("--cpu-sweeps", type=int, default=40)
+    ap.add_argument("--cpu-sweeps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serving-rates", default="200",
@@ -278,7 +279,7 @@ def main():
     if args.bisect_rounds and not args.only:
         bisection = run_bisection(args.bisect_rounds)
         log(f"bench: bisection Kc = {bisection['Jc']:.7f} +- {bisection['Jc_err']:.7f} (stat) "
-            f"+- {bisection['Jc_sys']:.7f} (levels) "
+            f"+- {bisection['Jc_sys']:.7f} (deepest two levels) "
             f"({bisection['updates_per_ns']:.1f} updates/ns, {bisection['seconds']:.1f} s)")
 
     import mcrg_amd

@@ -414,10 +414,14 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
     const int nl_small = n_match - 1, nl_large = k + nl_small, deep = n_match - 1;
     if (nl_large > 1 && L_large > 2048)
         throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "L_large above 2048 allows one blocking only");
-    const int spb = std::max(1, n_samples / std::max(2, bins_));
-    const int nb = (n_samples + spb - 1) / spb;
     const int rs_l = mcrg_record_size(nl_large), rs_s = mcrg_record_size(nl_small);
     const int per_l = n_large / n_k, per_s = n_small / n_k;
+    // jackknife groups of independent chains: replica r of every coupling and
+    // size falls in group r % G (chains, unlike bins of one chain, are
+    // independent however slowly the sampler decorrelates)
+    const int G = std::max(2, std::min(std::min(per_l, per_s), 16));
+    if (per_l < 2 || per_s < 2)
+        throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "need at least two lattices of each size per coupling");
     Bisection out;
     const auto t0 = std::chrono::steady_clock::now();
     double lo = J_lo, hi = J_hi;
@@ -430,55 +434,55 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
             const double J = lo + (hi - lo) * (q + 0.5) / n_k;
             br.J.push_back(J);
             for (int r = 0; r < per_l; ++r)
-                jobs.push_back({L_large, J, mcrg_cpp::next_seed(), n_samples_eq, n_samples, nl_large, spb, sampler_large});
+                jobs.push_back({L_large, J, mcrg_cpp::next_seed(), n_samples_eq, n_samples, nl_large, n_samples,
+                                sampler_large});
             for (int r = 0; r < per_s; ++r)
-                jobs.push_back({L_small, J, mcrg_cpp::next_seed(), n_samples_eq_small, n_samples, nl_small, spb,
+                jobs.push_back({L_small, J, mcrg_cpp::next_seed(), n_samples_eq_small, n_samples, nl_small, n_samples,
                                 MCRG_SAMPLER_METROPOLIS});
         }
         mcrg_batch b = nullptr;
         check(mcrg_batch_create(mcrg_cpp::context(), jobs.data(), (int32_t)jobs.size(), nullptr, &b));
         int32_t left = 1;
         while (left > 0) check(mcrg_batch_run(b, 1 << 30, &left));
-        // per coupling and bin: NN sums at the matched levels and sample counts
-        std::vector<double> SL((size_t)n_k * nb * n_match, 0.0), SS((size_t)n_k * nb * n_match, 0.0);
-        std::vector<double> NL((size_t)n_k * nb, 0.0), NS((size_t)n_k * nb, 0.0);
+        // per coupling and group: NN sums at the matched levels and sample counts
+        std::vector<double> SL((size_t)n_k * G * n_match, 0.0), SS((size_t)n_k * G * n_match, 0.0);
+        std::vector<double> NL((size_t)n_k * G, 0.0), NS((size_t)n_k * G, 0.0);
         std::vector<int64_t> rec;
         for (size_t j = 0; j < jobs.size(); ++j) {
             const int q = (int)(j / (size_t)(per_l + per_s));
-            const bool large = (int)(j % (size_t)(per_l + per_s)) < per_l;
+            const int r = (int)(j % (size_t)(per_l + per_s));
+            const bool large = r < per_l;
+            const int g = (large ? r : r - per_l) % G;
             const int nl = large ? nl_large : nl_small, rs = large ? rs_l : rs_s;
-            rec.assign((size_t)nb * rs, 0);
+            rec.assign((size_t)rs, 0);
             check(mcrg_batch_get_records(b, (int32_t)j, rec.data(), (int64_t)rec.size()));
-            for (int bin = 0; bin < nb; ++bin) {
-                const int64_t *r = rec.data() + (size_t)bin * rs;
-                for (int m = 0; m < n_match; ++m) {
-                    const int lvl = large ? k + m : m;
-                    (large ? SL : SS)[((size_t)q * nb + bin) * n_match + m] += (double)r[lvl * 3];
-                }
-                (large ? NL : NS)[(size_t)q * nb + bin] += (double)r[12 * (nl + 1) + 10 * nl + 3];
+            for (int m = 0; m < n_match; ++m) {
+                const int lvl = large ? k + m : m;
+                (large ? SL : SS)[((size_t)q * G + g) * n_match + m] += (double)rec[(size_t)lvl * 3];
             }
+            (large ? NL : NS)[(size_t)q * G + g] += (double)rec[(size_t)(12 * (nl + 1) + 10 * nl + 3)];
             out.site_updates += (double)jobs[j].L * jobs[j].L * (double)(jobs[j].n_therm + jobs[j].n_meas);
         }
         mcrg_batch_destroy(b);
-        // D per coupling (NN per site at the matched size), all bins and each bin left out
+        // D per coupling (NN per site at the matched size), all groups and each group left out
         auto D_of = [&](int q, int m, int skip) {
             double sl = 0, ss = 0, nl_ = 0, ns_ = 0;
-            for (int bin = 0; bin < nb; ++bin) {
-                if (bin == skip) continue;
-                sl += SL[((size_t)q * nb + bin) * n_match + m];
-                ss += SS[((size_t)q * nb + bin) * n_match + m];
-                nl_ += NL[(size_t)q * nb + bin];
-                ns_ += NS[(size_t)q * nb + bin];
+            for (int g = 0; g < G; ++g) {
+                if (g == skip) continue;
+                sl += SL[((size_t)q * G + g) * n_match + m];
+                ss += SS[((size_t)q * G + g) * n_match + m];
+                nl_ += NL[(size_t)q * G + g];
+                ns_ += NS[(size_t)q * G + g];
             }
             const double sites = (double)(L_small >> m) * (double)(L_small >> m);
             return (sl / nl_ - ss / ns_) / sites;
         };
-        const double jk = (double)(nb - 1) / nb;
+        const double jk = (double)(G - 1) / G;
         std::vector<double> w((size_t)n_k);
         for (int q = 0; q < n_k; ++q) {
             const double d = D_of(q, deep, -1);
             double var = 0.0;
-            for (int bin = 0; bin < nb; ++bin) var += (D_of(q, deep, bin) - d) * (D_of(q, deep, bin) - d);
+            for (int g = 0; g < G; ++g) var += (D_of(q, deep, g) - d) * (D_of(q, deep, g) - d);
             var *= jk;
             br.D.push_back(d);
             br.D_err.push_back(std::sqrt(var));
@@ -493,8 +497,8 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
         }
         double var = 0.0;
         std::vector<double> Dq((size_t)n_k);
-        for (int bin = 0; bin < nb; ++bin) {
-            for (int q = 0; q < n_k; ++q) Dq[(size_t)q] = D_of(q, deep, bin);
+        for (int g = 0; g < G; ++g) {
+            for (int q = 0; q < n_k; ++q) Dq[(size_t)q] = D_of(q, deep, g);
             const double c = crossing(br.J, Dq, w);
             var += (c - br.Jc) * (c - br.Jc);
         }
@@ -504,9 +508,17 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
                      br.Jc_err);
         out.rounds.push_back(br);
         if (!std::isfinite(br.Jc)) throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "no crossing: D(J) has no slope");
-        const double half = std::max(3.0 * br.Jc_err, (hi - lo) / 16.0);
-        lo = std::max(1e-6, br.Jc - half);
-        hi = br.Jc + half;
+        // next bracket: a quarter of the width around the crossing, never below
+        // three standard errors; a crossing outside the bracket moves it by at
+        // most one width without shrinking it
+        const double width = hi - lo;
+        double centre = br.Jc, half = std::max(3.0 * br.Jc_err, width / 8.0);
+        if (br.Jc < lo || br.Jc > hi) {
+            centre = std::min(std::max(br.Jc, lo - width / 2.0), hi + width / 2.0);
+            half = width / 2.0;
+        }
+        lo = std::max(1e-6, centre - half);
+        hi = centre + half;
     }
     out.Jc = out.rounds.back().Jc;
     out.Jc_err = out.rounds.back().Jc_err;

@@ -165,9 +165,10 @@ class MonteCarloRenormalizationGroup {
     // m times (k = log2(L_large / L_small); matched levels m = 0..3 while
     // L_small / 2^m >= 16: same final size).  D(J) = large - small at the
     // deepest matched level changes sign at the critical coupling; a weighted
-    // straight line through D(J) gives the crossing J0 and its delete-one-bin
-    // jackknife error, and the next bracket is J0 +- max(3 err, width / 16).
-    // J = |K|.
+    // straight line through D(J) gives the crossing J0 and its jackknife error
+    // over groups of independent chains, and the next bracket is
+    // J0 +- max(3 err, width / 8) (a crossing outside the bracket moves it by at
+    // most one width).  J = |K|.
     struct BisectionRound {
         double J_lo = 0, J_hi = 0, Jc = 0, Jc_err = 0;
         std::vector<double> J, D, D_err;

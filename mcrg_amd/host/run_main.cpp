@@ -101,14 +101,10 @@ int main(int argc, char **argv)
             const auto r = MCRG.bisect_critical_point(bisect, n_k, J_lo, J_hi, n_large, L_large, n_small, L_small,
                                                       n_eq, n, sampler_large, n_eq_small);
             const double exact = std::log(1.0 + std::sqrt(2.0)) / 2.0;
-            // systematic: spread of the crossings over the matched levels of the last round
+            // systematic: change of the crossing between the two deepest matched levels
             const auto &lv0 = r.rounds.back().Jc_level;
-            double lo_c = r.Jc, hi_c = r.Jc;
-            for (double c : lv0) {
-                lo_c = std::min(lo_c, c);
-                hi_c = std::max(hi_c, c);
-            }
-            const double sys = 0.5 * (hi_c - lo_c), tot = std::sqrt(r.Jc_err * r.Jc_err + sys * sys);
+            const double sys = lv0.size() >= 2 ? std::fabs(lv0[lv0.size() - 1] - lv0[lv0.size() - 2]) : 0.0;
+            const double tot = std::sqrt(r.Jc_err * r.Jc_err + sys * sys);
             std::printf("{\"Jc\": %.10f, \"Jc_err\": %.10f, \"Jc_sys\": %.10f, \"exact\": %.10f, "
                         "\"deviation\": %.10f, \"deviation_sigma\": %.3f, "
                         "\"rounds\": %i, \"seconds\": %.3f, \"site_updates\": %.6e, \"updates_per_ns\": %.3f, "
