@@ -1,0 +1,64 @@
+"""Physics parity with the reference's published MCRG results.
+
+The reference publishes nu = ln 2 / ln lambda per blocking level for N = 16 at
+the exact critical coupling, NN + NNN operators, b = 2 majority blocking
+(slides/MCRG.pdf p.22, from 1e7 Wolff samples):
+    n = 0: 1.0371936855,  n = 1: 1.0058615220,  n = 2: 0.9832937022.
+These are equilibrium expectations of the estimator, so any correct sampler
+reproduces them within statistical error.  The reference states no error for
+them; its 1e7 samples put them near 3e-4 (our own 2e7-sample runs at L = 16
+have jackknife errors of 3e-4), so the bounds below are about 6 combined
+errors for Swendsen-Wang and 5 of our jackknife errors for Metropolis, whose
+samples are correlated over ~L^2.2 sweeps.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KC = 0.5 * math.log(1.0 + math.sqrt(2.0))
+NU_PUBLISHED_N16 = (1.0371936855, 1.0058615220, 0.9832937022)
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import mcrg_amd
+    return mcrg_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(amd):
+    c = amd.Context(0)
+    yield c
+    c.close()
+
+
+def _nu(amd, ctx, sampler, n_rep, n_therm, n_meas, groups):
+    jobs = [dict(L=16, coupling=KC, seed=90000 + i, n_therm=n_therm, n_meas=n_meas, n_levels=3,
+                 sweeps_per_bin=n_meas, sampler=sampler) for i in range(n_rep)]
+    with amd.Batch(ctx, jobs) as b:
+        b.run_all()
+        recs = np.stack([b.records(j)[0] for j in range(n_rep)])
+    # independent replicas summed into jackknife groups
+    grp = recs.reshape(groups, n_rep // groups, -1).sum(axis=1)
+    return [amd.rg_exponent(grp, 3, lvl, 2) for lvl in range(3)]
+
+
+def test_nu_swendsen_wang_matches_published_n16(amd, ctx):
+    print("physics: swendsen-wang L=16", flush=True)
+    res = _nu(amd, ctx, amd.SAMPLER_SWENDSEN_WANG, 2048, 500, 10000, 64)
+    for lvl, (r, pub) in enumerate(zip(res, NU_PUBLISHED_N16)):
+        print(f"level {lvl}: nu = {r['nu']:.5f} +- {r['nu_err']:.5f} (published {pub:.5f})", flush=True)
+        assert 0 < r["nu_err"] < 0.001
+        assert abs(r["nu"] - pub) < 0.0025, (lvl, r["nu"], pub)
+
+
+def test_nu_metropolis_batched_matches_published_n16(amd, ctx):
+    print("physics: metropolis L=16 (LDS-resident)", flush=True)
+    res = _nu(amd, ctx, amd.SAMPLER_METROPOLIS, 1024, 10000, 50000, 64)
+    for lvl, (r, pub) in enumerate(zip(res, NU_PUBLISHED_N16)):
+        print(f"level {lvl}: nu = {r['nu']:.5f} +- {r['nu_err']:.5f} (published {pub:.5f})", flush=True)
+        assert 0 < r["nu_err"] < 0.004
+        assert abs(r["nu"] - pub) < 5 * r["nu_err"] + 0.0005, (lvl, r["nu"], pub, r["nu_err"])
