@@ -189,6 +189,10 @@ typedef struct {
                                   0 = 32 KiB per 256 threads                    */
     int32_t min_team;          /* lanes per lattice at least (1..threads, power
                                   of two); 0 = 16                               */
+    int32_t max_team;          /* lanes per lattice at most (power of two,
+                                  >= min_team); 0 = threads                     */
+    int32_t threads_large;     /* workgroup size for lattices of L = 512
+                                  (64/128/256/512); 0 = threads                 */
 } mcrg_batch_opts;
 
 mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n_jobs,
@@ -209,7 +213,11 @@ mcrg_status mcrg_batch_get_state(mcrg_batch b, int32_t job, uint32_t *sweep, uin
 /* Continuous batching of MCRG jobs (mcrg_job_desc) on one device.  Jobs are
  * submitted at any time from any thread; a scheduler admits queued jobs in
  * arrival order while their spins and records fit the device-memory budget and
- * their slot share of LDS fits the LDS budget, advances every admitted job by
+ * their residency share fits the LDS budget (a job's share is its slot's LDS,
+ * or the LDS per workgroup of a CU filled to its register-limited workgroup
+ * count, whichever is larger, divided by the lattices per slot; with the
+ * default budget every admitted job is resident in every launch, so a launch
+ * lasts about one time slice), advances every admitted job by
  * one time slice per launch of the LDS-resident schedule (jobs of different L
  * and coupling share each launch; a slot of G lattices of side L runs
  * min(sweeps_per_launch, max(1, launch_quantum / (G L^2))) sweeps, so every
@@ -223,13 +231,14 @@ typedef struct mcrg_server_s *mcrg_server;
 
 typedef struct {
     int64_t hbm_budget_bytes;  /* device memory for resident jobs; 0 = 4 GiB          */
-    int64_t lds_budget_bytes;  /* summed LDS share of resident jobs; 0 = two full
-                                  waves of workgroups (2 x CUs x LDS per CU)          */
+    int64_t lds_budget_bytes;  /* summed residency share of resident jobs (LDS
+                                  bytes, see below); 0 = one full wave of
+                                  workgroups (CUs x LDS per CU)                       */
     int64_t launch_quantum;    /* site updates per workgroup per launch; 0 = 2^24     */
     int32_t max_jobs;          /* jobs resident at once; 0 = 65536                    */
     int32_t threads;           /* workgroup size; 0 = 256                             */
     int32_t sweeps_per_launch; /* most sweeps of one job per launch; 0 = 65536        */
-    int32_t reserved;          /* 0                                                   */
+    int32_t threads_large;     /* workgroup size for L = 512 jobs; 0 = 512            */
 } mcrg_server_opts;
 
 enum { MCRG_JOB_QUEUED = 0, MCRG_JOB_RUNNING = 1, MCRG_JOB_DONE = 2, MCRG_JOB_CANCELLED = 3 };

@@ -55,7 +55,8 @@ class JobDesc(ctypes.Structure):
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int32), ("sweeps_per_launch", ctypes.c_int32),
-                ("max_group", ctypes.c_int32), ("max_slot_lds", ctypes.c_int32), ("min_team", ctypes.c_int32)]
+                ("max_group", ctypes.c_int32), ("max_slot_lds", ctypes.c_int32), ("min_team", ctypes.c_int32),
+                ("max_team", ctypes.c_int32), ("threads_large", ctypes.c_int32)]
 
 
 class RGResult(ctypes.Structure):
@@ -66,7 +67,7 @@ class RGResult(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("lds_budget_bytes", ctypes.c_int64),
                 ("launch_quantum", ctypes.c_int64), ("max_jobs", ctypes.c_int32), ("threads", ctypes.c_int32),
-                ("sweeps_per_launch", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("sweeps_per_launch", ctypes.c_int32), ("threads_large", ctypes.c_int32)]
 
 
 class JobStatus(ctypes.Structure):
@@ -319,12 +320,13 @@ class Batch:
     (mcrg_batch).  `jobs` is a list of dicts with keys L, coupling, seed,
     n_therm, n_meas, n_levels, sweeps_per_bin."""
 
-    def __init__(self, ctx, jobs, threads=0, sweeps_per_launch=0, max_group=0, max_slot_lds=0, min_team=0):
+    def __init__(self, ctx, jobs, threads=0, sweeps_per_launch=0, max_group=0, max_slot_lds=0, min_team=0,
+                 max_team=0, threads_large=0):
         arr = (JobDesc * len(jobs))()
         for i, j in enumerate(jobs):
             arr[i] = JobDesc(j["L"], float(j["coupling"]), j["seed"], j.get("n_therm", 0), j.get("n_meas", 0),
                              j.get("n_levels", 0), j.get("sweeps_per_bin", 1), j.get("sampler", 0))
-        opts = BatchOpts(threads, sweeps_per_launch, max_group, max_slot_lds, min_team)
+        opts = BatchOpts(threads, sweeps_per_launch, max_group, max_slot_lds, min_team, max_team, threads_large)
         h = ctypes.c_void_p()
         check(_lib.mcrg_batch_create(ctx.handle, arr, len(jobs), ctypes.byref(opts), ctypes.byref(h)))
         self.handle = h
@@ -380,9 +382,9 @@ class Server:
     this thread, or start() a scheduler thread of its own."""
 
     def __init__(self, ctx, hbm_budget_bytes=0, lds_budget_bytes=0, launch_quantum=0, max_jobs=0, threads=0,
-                 sweeps_per_launch=0):
+                 sweeps_per_launch=0, threads_large=0):
         opts = ServerOpts(hbm_budget_bytes, lds_budget_bytes, launch_quantum, max_jobs, threads, sweeps_per_launch,
-                          0)
+                          threads_large)
         h = ctypes.c_void_p()
         check(_lib.mcrg_server_create(ctx.handle, ctypes.byref(opts), ctypes.byref(h)))
         self.handle = h

@@ -74,7 +74,15 @@ mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n
         MCRG_REQUIRE(opts->max_slot_lds >= 0, "max_slot_lds must be >= 0");
         eo.max_slot_lds = (uint32_t)opts->max_slot_lds;
         if (opts->min_team) eo.min_team = opts->min_team;
+        eo.max_team = opts->max_team;
+        eo.threads_large = opts->threads_large;
     }
+    MCRG_REQUIRE(eo.threads_large == 0 || eo.threads_large == 64 || eo.threads_large == 128 ||
+                     eo.threads_large == 256 || eo.threads_large == 512,
+                 "threads_large must be 0, 64, 128, 256 or 512");
+    MCRG_REQUIRE(eo.max_team == 0 || (eo.max_team >= eo.min_team && eo.max_team <= eo.threads &&
+                                      (eo.max_team & (eo.max_team - 1)) == 0),
+                 "max_team must be 0 or a power of two in [min_team, threads]");
     MCRG_REQUIRE(eo.min_team >= 1 && eo.min_team <= eo.threads && (eo.min_team & (eo.min_team - 1)) == 0,
                  "min_team must be a power of two in [1, threads]");
     MCRG_REQUIRE(eo.threads == 64 || eo.threads == 128 || eo.threads == 256 || eo.threads == 512,

@@ -89,7 +89,11 @@ __device__ __forceinline__ uint32_t metropolis_word(uint32_t s, uint32_t nu, uin
     uint32_t accept = 0;
     if (open != 0u) {
         const uint32_t tag = (kTagSweep << 28) | (colour << 24);
+        // the first refinement call is drawn beside the first-stage call: a
+        // wave nearly always needs it, and the two independent Philox chains
+        // interleave
         const u32x4 r = philox(word, t, 0u, tag, k0, k1);
+        u32x4 ref = philox(word, t, 1u, tag, k0, k1);
         const uint32_t rq[4] = {r.v0, r.v1, r.v2, r.v3};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -99,11 +103,10 @@ __device__ __forceinline__ uint32_t metropolis_word(uint32_t s, uint32_t nu, uin
             accept |= open & ~rq[q] & tb;
             open &= ~(rq[q] ^ tb);
         }
-        u32x4 ref = {0u, 0u, 0u, 0u};
         for (uint32_t rank = 0; open != 0u; ++rank) {
             const uint32_t i = (uint32_t)__builtin_ctz(open);
             open &= open - 1u;
-            if ((rank & 3u) == 0u) ref = philox(word, t, 1u + (rank >> 2), tag, k0, k1);
+            if (rank != 0u && (rank & 3u) == 0u) ref = philox(word, t, 1u + (rank >> 2), tag, k0, k1);
             const uint32_t low = pick4(ref, rank & 3u) >> 4;
             const uint32_t thr = ((m1 >> i) & 1u) ? acc.thr0 : acc.thr1;
             accept |= (low < (thr & 0x0FFFFFFFu) ? 1u : 0u) << i;

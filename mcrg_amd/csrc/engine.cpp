@@ -100,13 +100,19 @@ hipError_t ResidentEngine::init(int device, size_t arena_bytes, int capacity, co
     hipError_t e;
     if ((e = arena_.init(arena_bytes)) != hipSuccess) return e;
     if ((e = hipMalloc(&d_lats_, sizeof(PackedLattice) * (size_t)capacity)) != hipSuccess) return e;
-    if ((e = hipMalloc(&d_queue_, 256)) != hipSuccess) return e;
+    if ((e = hipMalloc(&d_queue_, 64u * kMaxClasses)) != hipSuccess) return e;
     if ((e = hipMalloc(&d_init_idx_, sizeof(int32_t) * (size_t)capacity)) != hipSuccess) return e;
     for (hipStream_t &a : aux_)
         if ((e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking)) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming)) != hipSuccess) return e;
     for (hipEvent_t &ev : ev_join_)
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
+    int lds_cu = 0;
+    if ((e = hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device)) !=
+        hipSuccess)
+        return e;
+    lds_per_cu_ = (uint32_t)lds_cu;
     return hipSuccess;
 }
 
@@ -137,21 +143,23 @@ size_t ResidentEngine::job_bytes(const JobSpec &j)
 
 uint32_t ResidentEngine::lds_bytes(int L, int G, int nl) const
 {
-    const uint32_t T = (uint32_t)opt_.threads / (uint32_t)G;
+    const uint32_t T = (uint32_t)threads_for(L) / (uint32_t)G;
     return slot_layout((uint32_t)L, (uint32_t)G, (uint32_t)nl, T).total * 4u;
 }
 
 int ResidentEngine::group_size(int L, int nl) const
 {
     const int W = (int)cb_words((uint32_t)L);
-    int G = W >= opt_.threads ? 1 : opt_.threads / W;
+    const int NT = threads_for(L);
+    int G = W >= NT ? 1 : NT / W;
+    if (opt_.max_team > 0) G = std::max(G, NT / opt_.max_team);
     int cap = 1;
     while (cap * 2 <= opt_.max_group) cap *= 2;  // positions are a power of two
     G = std::min(G, cap);
     // a team of at least min_team lanes shares each lattice's measurement
-    G = std::min(G, std::max(1, opt_.threads / std::max(1, opt_.min_team)));
+    G = std::min(G, std::max(1, NT / std::max(1, opt_.min_team)));
     // keep a slot's LDS modest so that several workgroups share a CU
-    const uint32_t soft = opt_.max_slot_lds ? opt_.max_slot_lds : 32768u * (uint32_t)opt_.threads / 256u;
+    const uint32_t soft = opt_.max_slot_lds ? opt_.max_slot_lds : 32768u * (uint32_t)NT / 256u;
     while (G > 1 && lds_bytes(L, G, nl) > soft) G /= 2;
     return std::max(G, 1);
 }
@@ -159,7 +167,13 @@ int ResidentEngine::group_size(int L, int nl) const
 uint32_t ResidentEngine::lds_share(int L, int nl) const
 {
     const int G = group_size(L, nl);
-    return (lds_bytes(L, G, nl) + (uint32_t)G - 1u) / (uint32_t)G;
+    const int NT = threads_for(L);
+    // workgroups of this size a CU holds when LDS does not limit them
+    int &per_cu = wg_per_cu_[NT >= 512 ? 3 : NT >= 256 ? 2 : NT >= 128 ? 1 : 0];
+    if (per_cu == 0) per_cu = std::max(1, resident_max_active_groups(NT, 1024u) / std::max(1, cus_));
+    const uint32_t floor_bytes = lds_per_cu_ / (uint32_t)per_cu;
+    const uint32_t slot = std::max(lds_bytes(L, G, nl), floor_bytes);
+    return (slot + (uint32_t)G - 1u) / (uint32_t)G;
 }
 
 uint32_t ResidentEngine::slot_sweeps(int L, int G, uint32_t chunk) const
@@ -298,6 +312,7 @@ void ResidentEngine::rebuild_slots(uint32_t chunk)
         Slot s;
         uint64_t work;
         uint32_t lds;
+        int threads;
     };
     std::vector<Pending> pend;
     slot_lats_.clear();
@@ -323,22 +338,40 @@ void ResidentEngine::rebuild_slots(uint32_t chunk)
                 const Job &j = jobs_[(size_t)id];
                 work += (uint64_t)L * (uint64_t)L * std::min<uint64_t>(ns, j.therm_left + j.meas_left);
             }
-            pend.push_back({s, work, lds_bytes(L, gp, nl)});
+            pend.push_back({s, work, lds_bytes(L, gp, nl), threads_for(L)});
         }
     }
-    // launch classes by workgroups per CU (most first); merge the tail into
-    // the last class when there are more than kMaxClasses
-    std::map<uint32_t, int> resident;                  // slot LDS -> workgroups resident on the device
-    std::map<int, int, std::greater<int>> class_of;  // workgroups resident -> class
-    for (const Pending &p : pend) {
-        auto it = resident.find(p.lds);
-        if (it == resident.end()) it = resident.emplace(p.lds, resident_max_active_groups(opt_.threads, p.lds)).first;
-        class_of[it->second] = 0;
+    // launch classes by (workgroup size, workgroups resident on the device):
+    // larger workgroups first, then most resident first; the tail beyond a
+    // workgroup size's share of kMaxClasses merges into its last class
+    using ClassKey = std::pair<int, int>;  // (threads, resident workgroups)
+    std::map<std::pair<int, uint32_t>, int> resident;  // (threads, slot LDS) -> resident workgroups
+    auto key_of = [&](const Pending &p) {
+        auto it = resident.find({p.threads, p.lds});
+        if (it == resident.end())
+            it = resident.emplace(std::make_pair(p.threads, p.lds), resident_max_active_groups(p.threads, p.lds)).first;
+        return ClassKey(p.threads, it->second);
+    };
+    auto key_less = [](const ClassKey &a, const ClassKey &b) { return a.first != b.first ? a.first > b.first : a.second > b.second; };
+    std::map<ClassKey, int, decltype(key_less)> class_of(key_less);
+    for (const Pending &p : pend) class_of[key_of(p)] = 0;
+    // at most kMaxClasses / (workgroup sizes in use) classes per workgroup size
+    const int n_sizes = opt_.threads_large > 0 && opt_.threads_large != opt_.threads ? 2 : 1;
+    const int per_size = kMaxClasses / n_sizes;
+    int c = -1, last_threads = -1, in_size = 0;
+    for (auto &kv : class_of) {
+        if (kv.first.first != last_threads) {
+            ++c;
+            in_size = 1;
+            last_threads = kv.first.first;
+        } else if (in_size < per_size) {
+            ++c;
+            ++in_size;
+        }
+        kv.second = c;
     }
-    int c = 0;
-    for (auto &kv : class_of) kv.second = std::min(c++, kMaxClasses - 1);
     std::vector<int> cls(pend.size());
-    for (size_t i = 0; i < pend.size(); ++i) cls[i] = class_of[resident[pend[i].lds]];
+    for (size_t i = 0; i < pend.size(); ++i) cls[i] = class_of[key_of(pend[i])];
     // within a class, longest slots first: the persistent grid pulls them in this order
     std::vector<size_t> order(pend.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
@@ -356,6 +389,7 @@ void ResidentEngine::rebuild_slots(uint32_t chunk)
         LaunchClass &lc = classes_.back();
         lc.count += 1;
         lc.lds = std::max(lc.lds, pend[i].lds);
+        lc.threads = pend[i].threads;
         slots_.push_back(pend[i].s);
     }
 }
@@ -383,7 +417,7 @@ hipError_t ResidentEngine::upload_slots()
             return e;
     }
     for (LaunchClass &lc : classes_) {
-        const int resident = resident_max_active_groups(opt_.threads, lc.lds);
+        const int resident = resident_max_active_groups(lc.threads, lc.lds);
         if (resident <= 0) {
             failed_ = "hipOccupancyMaxActiveBlocksPerMultiprocessor";
             return hipErrorInvalidConfiguration;
@@ -405,7 +439,7 @@ hipError_t ResidentEngine::launch_classes(KernelTimer *tm)
         hipStream_t s = c == 0 ? stream_ : aux_[c - 1];
         if (c > 0 && (e = hipStreamWaitEvent(s, ev_fork_, 0)) != hipSuccess) return e;
         // one queue counter per class, 64 bytes apart
-        if ((e = launch_resident(opt_.threads, d_slots_ + lc.first, lc.count, d_slot_lats_, d_lats_,
+        if ((e = launch_resident(lc.threads, d_slots_ + lc.first, lc.count, d_slot_lats_, d_lats_,
                                  d_queue_ + 16u * c, lc.lds, (uint32_t)lc.grid, s)) != hipSuccess) {
             failed_ = "launch_resident";
             return e;

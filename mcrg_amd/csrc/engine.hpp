@@ -48,6 +48,10 @@ struct EngineOptions {
     int sweeps_per_launch = 250;  // chunk length (the most sweeps any lattice runs per launch)
     int max_group = 256;          // upper bound on lattices per slot
     int min_team = 16;            // lanes per lattice at least (measurement parallelism)
+    int max_team = 0;             // lanes per lattice at most (0 = threads): smaller teams
+                                  // loop over more words and measure lattices side by side
+    int threads_large = 0;        // workgroup size for lattices of L >= large_min_L (0 = threads)
+    int large_min_L = 512;
     // LDS a slot of small lattices may take (G halves until it fits);
     // 0 = 32 KiB per 256 threads, which leaves room for five workgroups per CU.
     uint32_t max_slot_lds = 0;
@@ -94,8 +98,15 @@ class ResidentEngine {
     // Name of the HIP call behind the last error returned by this engine.
     const char *failed_call() const { return failed_; }
     int group_size(int L, int nl) const;
+    // Workgroup size of the slots of side L.
+    int threads_for(int L) const
+    {
+        return opt_.threads_large > 0 && L >= opt_.large_min_L ? opt_.threads_large : opt_.threads;
+    }
     uint32_t lds_bytes(int L, int G, int nl) const;
-    // LDS bytes one lattice of this shape takes in its slot (slot LDS / G).
+    // Share of one CU's residency that one lattice of this shape takes, in
+    // LDS bytes: its slot's LDS, or the LDS a CU leaves each workgroup when
+    // registers cap the workgroups per CU, whichever is larger, over G.
     uint32_t lds_share(int L, int nl) const;
 
   private:
@@ -115,10 +126,10 @@ class ResidentEngine {
     // launch class; classes launch concurrently on their own streams, each
     // with its own dynamic LDS size, so that a few large lattices do not cut
     // the occupancy of every small one.
-    static constexpr int kMaxClasses = 4;
+    static constexpr int kMaxClasses = 6;
     struct LaunchClass {
         uint32_t first = 0, count = 0, lds = 0;
-        int grid = 0;
+        int threads = 0, grid = 0;
     };
     std::vector<LaunchClass> classes_;
     hipStream_t aux_[kMaxClasses - 1] = {};
@@ -134,6 +145,9 @@ class ResidentEngine {
     std::vector<int> free_ids_;
     PackedLattice *d_lats_ = nullptr;
     int capacity_ = 0;
+    int cus_ = 1;
+    uint32_t lds_per_cu_ = 65536;
+    mutable int wg_per_cu_[4] = {0, 0, 0, 0};  // by workgroup size 64, 128, 256, 512
     // slot tables: host copies and device mirrors
     std::vector<Slot> slots_;
     std::vector<int32_t> slot_lats_;

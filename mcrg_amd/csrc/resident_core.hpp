@@ -223,11 +223,26 @@ __device__ __forceinline__ uint32_t pyramid_level(uint32_t L, uint32_t T, uint32
     return 0u;
 }
 
+// Row y+1 (mod Ll) for lane y.  A team of a whole wave rotates the wave by one
+// lane with DPP (no LDS round trip) and patches the wrap-around row; smaller
+// teams shuffle within their segment.
+__device__ __forceinline__ uint32_t row_next(uint32_t v, uint32_t y, uint32_t Ll, uint32_t width)
+{
+    if (width == 64u) {
+        const uint32_t n = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x134, 0xF, 0xF, false);  // wave_rol:1
+        const uint32_t first = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+        return y == Ll - 1u ? first : n;
+    }
+    return (uint32_t)__shfl((int)v, (int)((y + 1u) & (Ll - 1u)), (int)width);
+}
+
 // Levels pl..nl of one lattice in registers: lane y (< 64, within the team's
-// first wave) holds row y of the current level (rows of 64 bits in two words),
-// neighbour rows come from cross-lane shuffles, blocked rows are formed from
-// lanes 2by and 2by+1, and the counts of each level are summed with DPP.
-// `width` is the shuffle segment (the team, or 64 when the team spans waves).
+// first wave) holds row y of the current level (rows of 64 bits in two words).
+// The row below comes from row_next; lane 2by forms blocked row by from its own
+// row and the row below, and one shuffle moves it to lane by.  Per level the
+// NN and NNN counts travel packed in one register (each < 2^14 for rows of at
+// most 64 sites) and are summed with DPP.  `width` is the shuffle segment (the
+// team, or 64 when the team spans waves).
 __device__ __forceinline__ void pyramid(const uint32_t *lg, const uint32_t *cg, uint32_t *cntg, uint32_t L,
                                         uint32_t pl, uint32_t nl, uint32_t lvl_off, uint32_t coin_off,
                                         uint32_t y, uint32_t width)
@@ -244,55 +259,55 @@ __device__ __forceinline__ void pyramid(const uint32_t *lg, const uint32_t *cg, 
     }
     uint32_t coff = coin_off;
     for (uint32_t l = pl; l <= nl; ++l) {
+        const bool wide = Ll == 64u;
+        const uint32_t Lb = Ll >> 1;
+        // coin word of the blocks this lane forms (lane 2by forms blocked row by)
+        uint32_t coin = 0;
+        if (l < nl && y < Ll && (y & 1u) == 0u) {
+            const uint32_t bm = (y >> 1) * Lb;
+            coin = cg[coff + (bm >> 5)] >> (bm & 31u);
+        }
         // ---- operators of level l
-        const uint32_t yn = (y + 1u) & (Ll - 1u);
-        const uint32_t dlo = (uint32_t)__shfl((int)lo, (int)yn, (int)width);
-        const uint32_t dhi = (uint32_t)__shfl((int)hi, (int)yn, (int)width);
-        uint32_t dnn = 0, dnnn = 0, dp = 0;
+        const uint32_t dlo = row_next(lo, y, Ll, width);
+        const uint32_t dhi = wide ? row_next(hi, y, Ll, width) : 0u;
+        uint32_t nn_nnn = 0, dp = 0;
         if (y < Ll) {
-            if (Ll == 64u) {
+            if (wide) {
                 const uint32_t rlo = (lo >> 1) | (hi << 31), rhi = (hi >> 1) | (lo << 31);
                 const uint32_t glo = (dlo >> 1) | (dhi << 31), ghi = (dhi >> 1) | (dlo << 31);
-                dnn = (uint32_t)(popc(lo ^ rlo) + popc(hi ^ rhi) + popc(lo ^ dlo) + popc(hi ^ dhi));
-                dnnn = (uint32_t)(popc(lo ^ glo) + popc(hi ^ ghi) + popc(rlo ^ dlo) + popc(rhi ^ dhi));
+                const uint32_t dnn = (uint32_t)(popc(lo ^ rlo) + popc(hi ^ rhi) + popc(lo ^ dlo) + popc(hi ^ dhi));
+                const uint32_t dnnn =
+                    (uint32_t)(popc(lo ^ glo) + popc(hi ^ ghi) + popc(rlo ^ dlo) + popc(rhi ^ dhi));
+                nn_nnn = dnn | (dnnn << 16);
                 dp = (uint32_t)(popc(lo ^ rlo ^ dlo ^ glo) + popc(hi ^ rhi ^ dhi ^ ghi));
             } else {
                 const uint32_t r = Ll == 32u ? ((lo >> 1) | (lo << 31)) : rot_right_small(lo, Ll);
                 const uint32_t g = Ll == 32u ? ((dlo >> 1) | (dlo << 31)) : rot_right_small(dlo, Ll);
-                dnn = (uint32_t)(popc(lo ^ r) + popc(lo ^ dlo));
-                dnnn = (uint32_t)(popc(lo ^ g) + popc(r ^ dlo));
+                nn_nnn = (uint32_t)(popc(lo ^ r) + popc(lo ^ dlo)) | ((uint32_t)(popc(lo ^ g) + popc(r ^ dlo)) << 16);
                 dp = (uint32_t)popc(lo ^ r ^ dlo ^ g);
             }
         }
-        dnn = group_sum(dnn, width);
-        dnnn = group_sum(dnnn, width);
+        nn_nnn = group_sum(nn_nnn, width);
         dp = group_sum(dp, width);
         if (y == 0u) {
             uint32_t *c = cntg + l * 4u;
-            c[0] = dnn;
-            c[1] = dnnn;
+            c[0] = nn_nnn & 0xFFFFu;
+            c[1] = nn_nnn >> 16;
             c[2] = dp;
             c[3] = 0u;
         }
         if (l == nl) break;
-        // ---- blocking into level l + 1: lane by takes rows 2by and 2by + 1
-        const uint32_t Lb = Ll >> 1;
-        const uint32_t s0 = (2u * y) & (Ll - 1u), s1 = (2u * y + 1u) & (Ll - 1u);
-        const uint32_t r0lo = (uint32_t)__shfl((int)lo, (int)s0, (int)width);
-        const uint32_t r1lo = (uint32_t)__shfl((int)lo, (int)s1, (int)width);
-        const uint32_t r0hi = (uint32_t)__shfl((int)hi, (int)s0, (int)width);
-        const uint32_t r1hi = (uint32_t)__shfl((int)hi, (int)s1, (int)width);
-        uint32_t nlo = 0;
-        if (y < Lb) {
-            const uint32_t bm = y * Lb;
-            const uint32_t coin = cg[coff + (bm >> 5)] >> (bm & 31u);
-            if (Ll == 64u)
-                nlo = block16(r0lo, r1lo, coin & 0xFFFFu) | (block16(r0hi, r1hi, coin >> 16) << 16);
+        // ---- blocking into level l + 1
+        uint32_t blk = 0;
+        if (y < Ll && (y & 1u) == 0u) {
+            if (wide)
+                blk = block16(lo, dlo, coin & 0xFFFFu) | (block16(hi, dhi, coin >> 16) << 16);
             else
-                nlo = block16(r0lo, r1lo, coin) & ((1u << Lb) - 1u);
+                blk = block16(lo, dlo, coin) & ((1u << Lb) - 1u);
         }
+        const uint32_t nlo = (uint32_t)__shfl((int)blk, (int)((2u * y) & (width - 1u)), (int)width);
         coff += level_words(L, l + 1u);
-        lo = nlo;
+        lo = y < Lb ? nlo : 0u;
         hi = 0u;
         Ll = Lb;
     }

@@ -119,14 +119,16 @@ def test_invalid_jobs(amd, ctx, bad, msg):
     assert e.value.status == amd.ERR_INVALID_ARG and msg in str(e.value)
 
 
-def test_launch_classes_mixed_lds(amd, ctx, orc):
+@pytest.mark.parametrize("threads_large", [0, 512])
+def test_launch_classes_mixed_lds(amd, ctx, orc, threads_large):
     """Slots of very different LDS size (L = 8 groups next to single L = 512
-    lattices) launch as concurrent classes; results stay bit-exact."""
+    lattices) launch as concurrent classes, with L = 512 on 256- or 512-thread
+    workgroups; results stay bit-exact."""
     jobs = [dict(L=L, coupling=KC, seed=300 + i, n_therm=2, n_meas=5, n_levels=min(nl, int(math.log2(L)) - 1),
                  sweeps_per_bin=2)
             for i, (L, nl) in enumerate([(512, 7), (8, 1), (256, 6), (8, 0), (16, 2), (512, 3), (32, 3)] * 3)]
-    print("classes", len(jobs), flush=True)
-    with amd.Batch(ctx, jobs, sweeps_per_launch=3) as b:
+    print("classes", len(jobs), threads_large, flush=True)
+    with amd.Batch(ctx, jobs, sweeps_per_launch=3, threads_large=threads_large) as b:
         b.run_all()
         _check_vs_oracle(orc, b, jobs)
 
