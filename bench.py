@@ -191,8 +191,9 @@ def _serve(mcrg_amd, srv, stream, rate, duration, drain_limit, quantum):
             "p50_ms": pct(0.50), "p99_ms": pct(0.99), "max_ms": lat_ms[-1] if lat_ms else None,
             "updates_per_ns": updates / (t_end * 1e9), "wall_s": t_end, "last_arrival_s": t_last_arrival,
             "max_submit_lag_s": late, "launches": st["launches"], "busy_s": st["busy_s"],
+            "device_wait_s": st["device_wait_s"],
             "yt_median": sorted(yts)[len(yts) // 2] if yts else None, "yt_undefined": yt_fail[0],
-            "launch_quantum": quantum or (1 << 24)}
+            "launch_quantum_us": quantum or 4000}
 
 
 def cpu_baseline(jobs, sweeps, threads):

@@ -213,15 +213,14 @@ mcrg_status mcrg_batch_get_state(mcrg_batch b, int32_t job, uint32_t *sweep, uin
 /* Continuous batching of MCRG jobs (mcrg_job_desc) on one device.  Jobs are
  * submitted at any time from any thread; a scheduler admits queued jobs in
  * arrival order while their spins and records fit the device-memory budget and
- * their residency share fits the LDS budget (a job's share is its slot's LDS,
- * or the LDS per workgroup of a CU filled to its register-limited workgroup
- * count, whichever is larger, divided by the lattices per slot; with the
- * default budget every admitted job is resident in every launch, so a launch
- * lasts about one time slice), advances every admitted job by
+ * their share of LDS (slot LDS / lattices per slot) fits the LDS budget,
+ * advances every admitted job by
  * one time slice per launch of the LDS-resident schedule (jobs of different L
  * and coupling share each launch; a slot of G lattices of side L runs
- * min(sweeps_per_launch, max(1, launch_quantum / (G L^2))) sweeps, so every
- * workgroup does about launch_quantum site updates), and retires finished jobs
+ * min(sweeps_per_launch, max(1, launch_quantum / t_sweep)) sweeps, with the
+ * slot's measured-sweep time modelled as 8 us + 75 ns x (sites each of its
+ * lanes sweeps), so every workgroup works about launch_quantum), and retires
+ * finished jobs
  * between launches: their records are copied to host memory and their device
  * memory is reused by the next admissions.  A job's results are bit-identical
  * to the same job run alone (its own seed and counters).  No reference
@@ -231,10 +230,10 @@ typedef struct mcrg_server_s *mcrg_server;
 
 typedef struct {
     int64_t hbm_budget_bytes;  /* device memory for resident jobs; 0 = 4 GiB          */
-    int64_t lds_budget_bytes;  /* summed residency share of resident jobs (LDS
-                                  bytes, see below); 0 = one full wave of
-                                  workgroups (CUs x LDS per CU)                       */
-    int64_t launch_quantum;    /* site updates per workgroup per launch; 0 = 2^24     */
+    int64_t lds_budget_bytes;  /* summed LDS share of resident jobs; 0 = two full
+                                  waves of workgroups (2 x CUs x LDS per CU)          */
+    int64_t launch_quantum;    /* microseconds of work per workgroup and launch;
+                                  0 = 4000                                            */
     int32_t max_jobs;          /* jobs resident at once; 0 = 65536                    */
     int32_t threads;           /* workgroup size; 0 = 256                             */
     int32_t sweeps_per_launch; /* most sweeps of one job per launch; 0 = 65536        */
@@ -256,6 +255,7 @@ typedef struct {
     int64_t launches;
     double site_updates;   /* completed sweeps x L^2 over all jobs              */
     double busy_s;         /* time with at least one job running                */
+    double device_wait_s;  /* part of busy_s spent waiting for launches         */
     int64_t hbm_in_use;    /* bytes of the budget held by resident jobs         */
     int64_t lds_in_use;    /* LDS shares of the resident jobs                   */
 } mcrg_server_stats;

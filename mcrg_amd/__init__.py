@@ -79,6 +79,7 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_int64), ("admitted", ctypes.c_int64), ("completed", ctypes.c_int64),
                 ("cancelled", ctypes.c_int64), ("queued", ctypes.c_int64), ("running", ctypes.c_int64),
                 ("launches", ctypes.c_int64), ("site_updates", ctypes.c_double), ("busy_s", ctypes.c_double),
+                ("device_wait_s", ctypes.c_double),
                 ("hbm_in_use", ctypes.c_int64), ("lds_in_use", ctypes.c_int64)]
 
 

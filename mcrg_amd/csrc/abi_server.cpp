@@ -70,7 +70,7 @@ static mcrg_status server_step(mcrg_server s, int32_t *active)
     {
         std::lock_guard<std::mutex> lk(s->mu);
         for (size_t i = 0; i < s->running.size();) {
-            ServerJob &j = s->jobs[s->running[i]];
+            ServerJob &j = s->jobs.at(s->running[i]);  // running jobs are never erased
             if (j.cancel) {
                 s->engine.remove(j.engine_id);
                 s->lds_in_use -= j.lds;
@@ -85,7 +85,7 @@ static mcrg_status server_step(mcrg_server s, int32_t *active)
         }
         while (!s->queue.empty()) {
             const uint64_t id = s->queue.front();
-            ServerJob &j = s->jobs[id];
+            ServerJob &j = s->jobs.at(id);  // queued jobs are never erased
             if (j.cancel) {
                 j.state = MCRG_JOB_CANCELLED;
                 s->st.cancelled++;
@@ -121,46 +121,71 @@ static mcrg_status server_step(mcrg_server s, int32_t *active)
     KernelTimer *tm = s->ctx->timer.enabled ? &s->ctx->timer : nullptr;
     hipError_t e = s->engine.tick((uint32_t)s->eo.sweeps_per_launch, adv, tm);
     (void)adv;
+    const double t_wait = s->now();
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    const double t_done = s->now();
     if (e != hipSuccess) {
         set_error(std::string("server launch (") + s->engine.failed_call() + "): " + hipGetErrorString(e));
         return MCRG_ERR_DEVICE;
     }
-    // ---- retire finished jobs
-    std::vector<uint64_t> done;
-    for (uint64_t id : s->running) {
-        ServerJob &j = s->jobs[id];
-        if (s->engine.finished(j.engine_id)) done.push_back(id);
+    // ---- retire finished jobs.  The job table is shared with client threads
+    //      (submit inserts, release erases): it is only touched under `mu`;
+    //      device reads happen outside the lock.
+    struct Done {
+        uint64_t id;
+        int engine_id;
+        std::vector<int64_t> rec;
+        double updates;
+    };
+    std::vector<Done> done;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (uint64_t id : s->running) {
+            auto it = s->jobs.find(id);
+            if (it == s->jobs.end() || !s->engine.finished(it->second.engine_id)) continue;
+            const ServerJob &j = it->second;
+            Done d;
+            d.id = id;
+            d.engine_id = j.engine_id;
+            d.rec.resize((size_t)j.n_bins * record_size((uint32_t)j.spec.n_levels));
+            d.updates = (double)j.spec.L * j.spec.L * (double)(j.spec.n_therm + j.spec.n_meas);
+            done.push_back(std::move(d));
+        }
     }
-    for (uint64_t id : done) {
-        ServerJob &j = s->jobs[id];
-        std::vector<int64_t> rec((size_t)j.n_bins * record_size((uint32_t)j.spec.n_levels));
-        e = s->engine.read_records(j.engine_id, rec.data(), (int64_t)rec.size());
+    for (Done &d : done) {
+        e = s->engine.read_records(d.engine_id, d.rec.data(), (int64_t)d.rec.size());
         if (e != hipSuccess) {
             set_error(std::string("server readback: ") + hipGetErrorString(e));
             return MCRG_ERR_DEVICE;
         }
-        s->engine.remove(j.engine_id);
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->lds_in_use -= j.lds;
-        j.records.swap(rec);
-        j.state = MCRG_JOB_DONE;
-        s->completed.push_back(id);
-        j.t_finish = s->now();
-        s->st.completed++;
-        s->st.site_updates += (double)j.spec.L * j.spec.L * (double)(j.spec.n_therm + j.spec.n_meas);
+        s->engine.remove(d.engine_id);
     }
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        for (uint64_t id : done)
-            for (size_t i = 0; i < s->running.size(); ++i)
-                if (s->running[i] == id) {
-                    s->running[i] = s->running.back();
-                    s->running.pop_back();
-                    break;
-                }
+        const double t = s->now();
+        for (Done &d : done) {
+            auto it = s->jobs.find(d.id);
+            if (it == s->jobs.end() || it->second.state != MCRG_JOB_RUNNING) continue;
+            ServerJob &j = it->second;
+            s->lds_in_use -= j.lds;
+            j.records.swap(d.rec);
+            j.state = MCRG_JOB_DONE;
+            j.t_finish = t;
+            s->completed.push_back(d.id);
+            s->st.completed++;
+            s->st.site_updates += d.updates;
+        }
+        if (!done.empty()) {
+            size_t k = 0;
+            for (uint64_t id : s->running) {
+                auto it = s->jobs.find(id);
+                if (it != s->jobs.end() && it->second.state == MCRG_JOB_RUNNING) s->running[k++] = id;
+            }
+            s->running.resize(k);
+        }
         s->st.launches++;
         s->st.busy_s += s->now() - t_launch;
+        s->st.device_wait_s += t_done - t_wait;
         s->st.running = (int64_t)s->running.size();
         s->st.hbm_in_use = (int64_t)s->engine.arena_in_use();
         s->st.lds_in_use = s->lds_in_use;
@@ -199,12 +224,12 @@ mcrg_status mcrg_server_create(mcrg_ctx ctx, const mcrg_server_opts *opts, mcrg_
     int max_jobs = 65536;
     EngineOptions eo;
     eo.sweeps_per_launch = 65536;
-    eo.quantum = uint64_t(1) << 24;
+    eo.quantum_ns = 4000000;  // 4 ms of work per workgroup and launch
     eo.threads_large = 512;  // a single L = 512 lattice sweeps 1.5x faster on 512 lanes
     if (opts) {
         if (opts->hbm_budget_bytes) budget = opts->hbm_budget_bytes;
         if (opts->lds_budget_bytes) lds_budget = opts->lds_budget_bytes;
-        if (opts->launch_quantum) eo.quantum = (uint64_t)opts->launch_quantum;
+        if (opts->launch_quantum) eo.quantum_ns = (uint64_t)opts->launch_quantum * 1000u;
         if (opts->max_jobs) max_jobs = opts->max_jobs;
         if (opts->threads) eo.threads = opts->threads;
         if (opts->sweeps_per_launch) eo.sweeps_per_launch = opts->sweeps_per_launch;
@@ -225,7 +250,7 @@ mcrg_status mcrg_server_create(mcrg_ctx ctx, const mcrg_server_opts *opts, mcrg_
         int cus = 0, lds_cu = 0;
         MCRG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
         MCRG_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, ctx->device));
-        lds_budget = (int64_t)cus * (int64_t)lds_cu;
+        lds_budget = 2 * (int64_t)cus * (int64_t)lds_cu;
     }
     mcrg_server s = new (std::nothrow) mcrg_server_s;
     if (!s) {

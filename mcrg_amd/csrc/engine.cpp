@@ -107,12 +107,6 @@ hipError_t ResidentEngine::init(int device, size_t arena_bytes, int capacity, co
     if ((e = hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming)) != hipSuccess) return e;
     for (hipEvent_t &ev : ev_join_)
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
-    int lds_cu = 0;
-    if ((e = hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) return e;
-    if ((e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device)) !=
-        hipSuccess)
-        return e;
-    lds_per_cu_ = (uint32_t)lds_cu;
     return hipSuccess;
 }
 
@@ -167,19 +161,14 @@ int ResidentEngine::group_size(int L, int nl) const
 uint32_t ResidentEngine::lds_share(int L, int nl) const
 {
     const int G = group_size(L, nl);
-    const int NT = threads_for(L);
-    // workgroups of this size a CU holds when LDS does not limit them
-    int &per_cu = wg_per_cu_[NT >= 512 ? 3 : NT >= 256 ? 2 : NT >= 128 ? 1 : 0];
-    if (per_cu == 0) per_cu = std::max(1, resident_max_active_groups(NT, 1024u) / std::max(1, cus_));
-    const uint32_t floor_bytes = lds_per_cu_ / (uint32_t)per_cu;
-    const uint32_t slot = std::max(lds_bytes(L, G, nl), floor_bytes);
-    return (slot + (uint32_t)G - 1u) / (uint32_t)G;
+    return (lds_bytes(L, G, nl) + (uint32_t)G - 1u) / (uint32_t)G;
 }
 
 uint32_t ResidentEngine::slot_sweeps(int L, int G, uint32_t chunk) const
 {
-    if (opt_.quantum == 0) return chunk;
-    const uint64_t n = opt_.quantum / ((uint64_t)G * (uint64_t)L * (uint64_t)L);
+    if (opt_.quantum_ns == 0) return chunk;
+    const uint64_t sites_per_lane = (uint64_t)G * (uint64_t)L * (uint64_t)L / (uint64_t)threads_for(L);
+    const uint64_t n = opt_.quantum_ns / (kSweepFixedNs + kSweepPerSiteNs * sites_per_lane);
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n, chunk));
 }
 

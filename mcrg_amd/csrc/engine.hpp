@@ -43,6 +43,11 @@ class Arena {
     std::map<size_t, size_t> free_;  // offset -> size
 };
 
+// Measured-sweep time model of one slot on MI355X (from per-L runs of the
+// batched workload: about 8 us per sample plus 75 ns per site a lane sweeps).
+constexpr uint64_t kSweepFixedNs = 8000;
+constexpr uint64_t kSweepPerSiteNs = 75;
+
 struct EngineOptions {
     int threads = 256;           // workgroup size of the resident kernel
     int sweeps_per_launch = 250;  // chunk length (the most sweeps any lattice runs per launch)
@@ -55,10 +60,13 @@ struct EngineOptions {
     // LDS a slot of small lattices may take (G halves until it fits);
     // 0 = 32 KiB per 256 threads, which leaves room for five workgroups per CU.
     uint32_t max_slot_lds = 0;
-    // Time slicing: when nonzero, a slot of G positions of side L runs
-    // min(chunk, max(1, quantum / (G L^2))) sweeps per launch, so that every
-    // workgroup does about `quantum` site updates; 0 = every slot runs the chunk.
-    uint64_t quantum = 0;
+    // Time slicing: when nonzero, a slot runs min(chunk, max(1, quantum_ns /
+    // t_sweep)) sweeps per launch, with the measured-sweep time of a slot
+    // modelled as t_sweep = kSweepFixedNs + kSweepPerSiteNs * (G L^2 / threads)
+    // (a per-sample cost of the measurement and the sites each lane sweeps),
+    // so that every workgroup works about quantum_ns; 0 = every slot runs the
+    // chunk.
+    uint64_t quantum_ns = 0;
 };
 
 class ResidentEngine {
@@ -104,9 +112,7 @@ class ResidentEngine {
         return opt_.threads_large > 0 && L >= opt_.large_min_L ? opt_.threads_large : opt_.threads;
     }
     uint32_t lds_bytes(int L, int G, int nl) const;
-    // Share of one CU's residency that one lattice of this shape takes, in
-    // LDS bytes: its slot's LDS, or the LDS a CU leaves each workgroup when
-    // registers cap the workgroups per CU, whichever is larger, over G.
+    // LDS bytes one lattice of this shape takes in its slot (slot LDS / G).
     uint32_t lds_share(int L, int nl) const;
 
   private:
@@ -145,9 +151,7 @@ class ResidentEngine {
     std::vector<int> free_ids_;
     PackedLattice *d_lats_ = nullptr;
     int capacity_ = 0;
-    int cus_ = 1;
-    uint32_t lds_per_cu_ = 65536;
-    mutable int wg_per_cu_[4] = {0, 0, 0, 0};  // by workgroup size 64, 128, 256, 512
+
     // slot tables: host copies and device mirrors
     std::vector<Slot> slots_;
     std::vector<int32_t> slot_lats_;

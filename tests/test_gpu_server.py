@@ -47,7 +47,7 @@ def _expect(orc, job):
                          job["sweeps_per_bin"])[1]
 
 
-@pytest.mark.parametrize("chunk,quantum", [(7, 0), (0, 20000), (0, 1)])
+@pytest.mark.parametrize("chunk,quantum", [(7, 0), (0, 300), (0, 1)])
 def test_stream_bit_exact_vs_oracle(amd, ctx, orc, chunk, quantum):
     """Jobs arrive in three waves between launches; each equals the oracle,
     with uniform chunks and with time slices that differ per slot."""
