@@ -55,7 +55,12 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 KiB per dispatch, the scale factors read
 # off the same counters on a 1 GiB read + 1 GiB write copy kernel
 # (--copy-calibration).  Other shapes report null.
-MEASURED_TRAFFIC = {(4096, 100, 256): (3178.7 * 2.0 + 5028.8) * 1024.0}
+MEASURED_TRAFFIC = {(4096, 100, 256): (3151.5 * 2.0 + 5051.8) * 1024.0}
+# The same kernel is bound by VALU issue, not HBM: rocprofv3 --pmc SQ_INSTS_VALU
+# and GRBM_GUI_ACTIVE over the same launches give the wave-level VALU
+# instructions per launch and the busy cycles per XCD (8 XCDs, 1024 SIMDs that
+# issue one wave64 VALU instruction per 4 cycles).
+MEASURED_VALU = {(4096, 100, 256): {"valu_wave_instr_per_launch": 6.354e8, "gui_active_cycles_per_xcd": 35491179.6 / 8}}
 
 
 def log(*a):
@@ -214,6 +219,16 @@ def cpu_baseline(jobs, sweeps, threads):
                       f"measured sweeps each, {threads} threads over independent replicas: {dt:.1f} s"}
 
 
+def compute_line(replicas, sps, threads, sites):
+    m = MEASURED_VALU.get((replicas, sps, threads))
+    if not m:
+        return None
+    capacity = 1024 * m["gui_active_cycles_per_xcd"] / 4.0
+    return {"bound": "valu", "frac": m["valu_wave_instr_per_launch"] / capacity,
+            "lane_instr_per_update": 64.0 * m["valu_wave_instr_per_launch"] / (sites * sps),
+            "source": "rocprofv3 --pmc SQ_INSTS_VALU GRBM_GUI_ACTIVE on bench.py --only batched"}
+
+
 def run_bisection(rounds):
     import subprocess
     cmd = [os.path.join(ROOT, "mcrg_amd", "mcrg_run"), "--bisect", str(rounds), "--eq", "1000", "--eq-small",
@@ -255,7 +270,7 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--serving-rates", default="200",
+    ap.add_argument("--serving-rates", default="200,1000",
                     help="comma-separated arrival rates in jobs/s for the serving leg ('' = none)")
     ap.add_argument("--serving-duration", type=float, default=10.0)
     ap.add_argument("--serving-drain", type=float, default=120.0,
@@ -354,6 +369,7 @@ def main():
                      "traffic": MEASURED_TRAFFIC.get((len(jobs), args.sweeps_per_step, args.threads)),
                      "kernel": f"k_resident<{args.threads}>",
                      "avg_ms": avg_ms, "bytes_per_launch": bpl},
+        "compute": compute_line(len(jobs), args.sweeps_per_step, args.threads, sites),
         "plain_mixed": plain_mixed,
         "plain_l2048": plain_2048,
         "batched_over_plain_mixed": (value / plain_mixed) if plain_mixed else None,
