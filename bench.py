@@ -73,25 +73,31 @@ def bytes_per_update(measured):
     return 3.0 / 8.0 + (5.0 / 24.0 if measured else 0.0)
 
 
-def run_batched(mcrg_amd, ctx, jobs, steps, warmup, sps, threads):
+def run_batched(mcrg_amd, ctx, jobs, steps, warmup, sps, threads, autotune=False):
+    """The autotuner (when on) tries its candidates during the warmup steps."""
     n_sweeps = (warmup + steps) * sps
     bjobs = [dict(j, n_therm=0, n_meas=n_sweeps, sweeps_per_bin=max(1, n_sweeps)) for j in jobs]
     sites = sum(j["L"] * j["L"] for j in bjobs)
-    with mcrg_amd.Batch(ctx, bjobs, threads=threads, sweeps_per_launch=sps) as b:
+    with mcrg_amd.Batch(ctx, bjobs, threads=threads, sweeps_per_launch=sps, autotune=autotune) as b:
         b.run(warmup * sps)
         b.sync()
+        opts = b.options()
+        if opts["autotune"]:
+            raise RuntimeError("autotuning needs more warmup steps")
         t0 = time.perf_counter()
         b.run(steps * sps)
         b.sync()
         dt = time.perf_counter() - t0
-    return sites * sps * steps / (dt * 1e9), dt, sites
+    return sites * sps * steps / (dt * 1e9), dt, sites, opts
 
 
-def run_batched_timed(mcrg_amd, ctx, jobs, sps, threads, launches):
-    """Kernel duration of the resident kernel from HIP events on its stream."""
+def run_batched_timed(mcrg_amd, ctx, jobs, sps, tuned, launches):
+    """Kernel duration of the resident kernel from HIP events on its stream,
+    in the configuration `tuned` (the one the timed leg ran)."""
     n_sweeps = launches * sps
     bjobs = [dict(j, n_therm=0, n_meas=n_sweeps, sweeps_per_bin=max(1, n_sweeps)) for j in jobs]
-    with mcrg_amd.Batch(ctx, bjobs, threads=threads, sweeps_per_launch=sps) as b:
+    with mcrg_amd.Batch(ctx, bjobs, threads=tuned["threads"], sweeps_per_launch=sps, min_team=tuned["min_team"],
+                        max_slot_lds=tuned["max_slot_lds"], threads_large=tuned["threads_large"]) as b:
         b.run(sps)  # first launch builds the slot table
         b.sync()
         n0, ms0 = ctx.kernel_time(3)
@@ -219,8 +225,8 @@ def cpu_baseline(jobs, sweeps, threads):
                       f"measured sweeps each, {threads} threads over independent replicas: {dt:.1f} s"}
 
 
-def compute_line(replicas, sps, threads, sites):
-    m = MEASURED_VALU.get((replicas, sps, threads))
+def compute_line(cfg_key, sites, sps):
+    m = MEASURED_VALU.get(cfg_key)
     if not m:
         return None
     capacity = 1024 * m["gui_active_cycles_per_xcd"] / 4.0
@@ -304,7 +310,8 @@ def main():
     jobs = workloads.batch_manifest(args.replicas)
     ctx = mcrg_amd.Context(0)
     if args.only == "batched":
-        value, dt, _ = run_batched(mcrg_amd, ctx, jobs, args.steps, args.warmup, args.sweeps_per_step, args.threads)
+        value, dt, _, _ = run_batched(mcrg_amd, ctx, jobs, args.steps, args.warmup, args.sweeps_per_step,
+                                      args.threads)
         log(f"bench: batched {value:.1f} updates/ns ({dt:.3f} s timed)")
         ctx.close()
         return 0
@@ -315,11 +322,15 @@ def main():
     dev = ctx.device_info()
     log(f"bench: {dev['name']} {dev['compute_units']} CUs; {len(jobs)} replicas, "
         f"{args.warmup}+{args.steps} steps x {args.sweeps_per_step} sweeps")
-    value, dt, sites = run_batched(mcrg_amd, ctx, jobs, args.steps, args.warmup, args.sweeps_per_step,
-                                   args.threads)
-    log(f"bench: batched {value:.1f} updates/ns ({dt:.3f} s timed)")
+    value, dt, sites, tuned = run_batched(mcrg_amd, ctx, jobs, args.steps, args.warmup, args.sweeps_per_step,
+                                          args.threads, autotune=args.warmup >= 10)
+    log(f"bench: batched {value:.1f} updates/ns ({dt:.3f} s timed); configuration "
+        f"threads={tuned['threads']} min_team={tuned['min_team']} max_slot_lds={tuned['max_slot_lds']} "
+        f"threads_large={tuned['threads_large']}")
 
-    avg_ms, n_launch = run_batched_timed(mcrg_amd, ctx, jobs, args.sweeps_per_step, args.threads, 11)
+    avg_ms, n_launch = run_batched_timed(mcrg_amd, ctx, jobs, args.sweeps_per_step, tuned, 11)
+    default_cfg = (tuned["threads"], tuned["min_team"], tuned["max_slot_lds"], tuned["threads_large"]) == (256, 16, 0, 0)
+    cfg_key = (len(jobs), args.sweeps_per_step, 256) if default_cfg else None
     bpl = bytes_per_update(True) * sites * args.sweeps_per_step
     achieved = bpl / (avg_ms * 1e-3) / 1e9
     log(f"bench: resident kernel {avg_ms:.3f} ms/launch over {n_launch} launches -> {achieved:.1f} GB/s")
@@ -362,14 +373,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"batched: {len(jobs)} replicas, L in {{8..128}}, 5 couplings, seeds 0..{len(jobs)-1}, "
                                f"bit-packed LDS-resident, blocking to L=4, every sweep measured",
-                   "replicas": len(jobs), "sweeps_per_step": args.sweeps_per_step, "threads": args.threads,
+                   "replicas": len(jobs), "sweeps_per_step": args.sweeps_per_step, "threads": tuned["threads"],
+                   "autotuned": {k: tuned[k] for k in ("threads", "min_team", "max_slot_lds", "threads_large")},
                    "sites": sites, "layout": "packed", "schedule": "resident", "device": dev["name"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get((len(jobs), args.sweeps_per_step, args.threads)),
-                     "kernel": f"k_resident<{args.threads}>",
+                     "traffic": MEASURED_TRAFFIC.get(cfg_key),
+                     "kernel": f"k_resident<{tuned['threads']}>",
                      "avg_ms": avg_ms, "bytes_per_launch": bpl},
-        "compute": compute_line(len(jobs), args.sweeps_per_step, args.threads, sites),
+        "compute": compute_line(cfg_key, sites, args.sweeps_per_step),
         "plain_mixed": plain_mixed,
         "plain_l2048": plain_2048,
         "batched_over_plain_mixed": (value / plain_mixed) if plain_mixed else None,

@@ -193,6 +193,10 @@ typedef struct {
                                   >= min_team); 0 = threads                     */
     int32_t threads_large;     /* workgroup size for lattices of L = 512
                                   (64/128/256/512); 0 = threads                 */
+    int32_t autotune;          /* nonzero: the first launches each try one
+                                  candidate configuration (timed with events)
+                                  and the fastest is kept; results are the same
+                                  under every configuration                     */
 } mcrg_batch_opts;
 
 mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n_jobs,
@@ -208,6 +212,9 @@ mcrg_status mcrg_batch_get_records(mcrg_batch b, int32_t job, int64_t *host, int
 /* Spins of job j as +-1 int8, row-major, n = L*L. */
 mcrg_status mcrg_batch_get_spins(mcrg_batch b, int32_t job, int8_t *host, int64_t n);
 mcrg_status mcrg_batch_get_state(mcrg_batch b, int32_t job, uint32_t *sweep, uint32_t *n_measured);
+/* Configuration in use (after autotuning: the one it kept; autotune = 1 while
+ * candidates are still being tried).  No reference counterpart. */
+mcrg_status mcrg_batch_get_opts(mcrg_batch b, mcrg_batch_opts *out);
 
 /* ------------------------------------------------------------------ serving */
 /* Continuous batching of MCRG jobs (mcrg_job_desc) on one device.  Jobs are

@@ -56,7 +56,7 @@ class JobDesc(ctypes.Structure):
 class BatchOpts(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int32), ("sweeps_per_launch", ctypes.c_int32),
                 ("max_group", ctypes.c_int32), ("max_slot_lds", ctypes.c_int32), ("min_team", ctypes.c_int32),
-                ("max_team", ctypes.c_int32), ("threads_large", ctypes.c_int32)]
+                ("max_team", ctypes.c_int32), ("threads_large", ctypes.c_int32), ("autotune", ctypes.c_int32)]
 
 
 class RGResult(ctypes.Structure):
@@ -121,6 +121,7 @@ _SIGS = {
     "mcrg_batch_get_records": (_c.c_int32, [_p, _c.c_int32, _p, _c.c_int64]),
     "mcrg_batch_get_spins": (_c.c_int32, [_p, _c.c_int32, _p, _c.c_int64]),
     "mcrg_batch_get_state": (_c.c_int32, [_p, _c.c_int32, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint32)]),
+    "mcrg_batch_get_opts": (_c.c_int32, [_p, _c.POINTER(BatchOpts)]),
     "mcrg_server_create": (_c.c_int32, [_p, _c.POINTER(ServerOpts), _c.POINTER(_p)]),
     "mcrg_server_destroy": (_c.c_int32, [_p]),
     "mcrg_server_submit": (_c.c_int32, [_p, _c.POINTER(JobDesc), _c.POINTER(_c.c_uint64)]),
@@ -322,12 +323,13 @@ class Batch:
     n_therm, n_meas, n_levels, sweeps_per_bin."""
 
     def __init__(self, ctx, jobs, threads=0, sweeps_per_launch=0, max_group=0, max_slot_lds=0, min_team=0,
-                 max_team=0, threads_large=0):
+                 max_team=0, threads_large=0, autotune=False):
         arr = (JobDesc * len(jobs))()
         for i, j in enumerate(jobs):
             arr[i] = JobDesc(j["L"], float(j["coupling"]), j["seed"], j.get("n_therm", 0), j.get("n_meas", 0),
                              j.get("n_levels", 0), j.get("sweeps_per_bin", 1), j.get("sampler", 0))
-        opts = BatchOpts(threads, sweeps_per_launch, max_group, max_slot_lds, min_team, max_team, threads_large)
+        opts = BatchOpts(threads, sweeps_per_launch, max_group, max_slot_lds, min_team, max_team, threads_large,
+                         1 if autotune else 0)
         h = ctypes.c_void_p()
         check(_lib.mcrg_batch_create(ctx.handle, arr, len(jobs), ctypes.byref(opts), ctypes.byref(h)))
         self.handle = h
@@ -363,6 +365,12 @@ class Batch:
         t, q = ctypes.c_uint32(), ctypes.c_uint32()
         check(_lib.mcrg_batch_get_state(self.handle, j, ctypes.byref(t), ctypes.byref(q)))
         return t.value, q.value
+
+    def options(self):
+        """Configuration in use (the autotuner's choice once it has finished)."""
+        o = BatchOpts()
+        check(_lib.mcrg_batch_get_opts(self.handle, ctypes.byref(o)))
+        return {f: getattr(o, f) for f, _ in BatchOpts._fields_}
 
     def close(self):
         if self.handle:

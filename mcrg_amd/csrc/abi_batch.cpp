@@ -169,6 +169,7 @@ mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n
         if ((e = b->groups.back()->init(ms, b->hbm_stream)) != hipSuccess) return fail(e, "batch HBM group");
     }
     if ((e = b->engine.flush_pending()) != hipSuccess) return fail(e, "batch init");
+    if (opts && opts->autotune && n_resident > 0) b->engine.enable_autotune();
     *out = b;
     return MCRG_OK;
 }
@@ -262,6 +263,21 @@ mcrg_status mcrg_batch_get_spins(mcrg_batch b, int32_t job, int8_t *host, int64_
         MCRG_REQUIRE(n == (int64_t)g.spec().L * g.spec().L, "n must be L*L");
         MCRG_HIP(g.read_spins(r.index, host, b->hbm_stream));
     }
+    return MCRG_OK;
+}
+
+mcrg_status mcrg_batch_get_opts(mcrg_batch b, mcrg_batch_opts *out)
+{
+    MCRG_REQUIRE(b != nullptr && out != nullptr, "null argument");
+    const EngineOptions &o = b->engine.options();
+    out->threads = o.threads;
+    out->sweeps_per_launch = o.sweeps_per_launch;
+    out->max_group = o.max_group;
+    out->max_slot_lds = (int32_t)o.max_slot_lds;
+    out->min_team = o.min_team;
+    out->max_team = o.max_team;
+    out->threads_large = o.threads_large;
+    out->autotune = b->engine.tuning() ? 1 : 0;
     return MCRG_OK;
 }
 

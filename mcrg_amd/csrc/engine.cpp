@@ -80,6 +80,8 @@ ResidentEngine::~ResidentEngine()
             (void)hipStreamDestroy(a);
         }
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (tune_a_) (void)hipEventDestroy(tune_a_);
+    if (tune_b_) (void)hipEventDestroy(tune_b_);
     for (hipEvent_t e : ev_join_)
         if (e) (void)hipEventDestroy(e);
     arena_.release();
@@ -139,6 +141,22 @@ uint32_t ResidentEngine::lds_bytes(int L, int G, int nl) const
 {
     const uint32_t T = (uint32_t)threads_for(L) / (uint32_t)G;
     return slot_layout((uint32_t)L, (uint32_t)G, (uint32_t)nl, T).total * 4u;
+}
+
+void ResidentEngine::enable_autotune()
+{
+    // the first entry repeats the configuration in use to absorb first-launch costs
+    const EngineOptions o = opt_;
+    tune_ = {{o.threads, o.min_team, o.threads_large, o.max_slot_lds, 0.0},
+             {o.threads, o.min_team, o.threads_large, o.max_slot_lds, 0.0},
+             {128, 16, 0, 0, 0.0},
+             {256, 16, 0, 0, 0.0},
+             {512, 16, 0, 0, 0.0},
+             {256, 32, 0, 0, 0.0},
+             {256, 8, 0, 0, 0.0},
+             {256, 16, 512, 0, 0.0},
+             {256, 16, 0, 65536, 0.0}};
+    tune_next_ = 0;
 }
 
 int ResidentEngine::group_size(int L, int nl) const
@@ -453,6 +471,17 @@ hipError_t ResidentEngine::tick(uint32_t n, uint32_t &advanced, KernelTimer *tm)
     }
     const uint32_t chunk = std::min<uint32_t>(n, (uint32_t)opt_.sweeps_per_launch);
     if (chunk == 0 || max_remaining() == 0) return hipSuccess;
+    const bool timing = tuning();
+    if (timing) {
+        const TuneCandidate &c = tune_[tune_next_];
+        opt_.threads = c.threads;
+        opt_.min_team = std::min(c.min_team, c.threads);
+        opt_.threads_large = c.threads_large;
+        opt_.max_slot_lds = c.max_slot_lds;
+        slots_dirty_ = true;
+        if (!tune_a_ && (e = hipEventCreate(&tune_a_)) != hipSuccess) return e;
+        if (!tune_b_ && (e = hipEventCreate(&tune_b_)) != hipSuccess) return e;
+    }
     if (slots_dirty_ || chunk != slots_chunk_) {
         rebuild_slots(chunk);
         if ((e = upload_slots()) != hipSuccess) {
@@ -463,12 +492,15 @@ hipError_t ResidentEngine::tick(uint32_t n, uint32_t &advanced, KernelTimer *tm)
         slots_chunk_ = chunk;
     }
     if (slots_.empty()) return hipSuccess;
+    if (timing && (e = hipEventRecord(tune_a_, stream_)) != hipSuccess) return e;
     if ((e = launch_classes(tm)) != hipSuccess) {
         if (!*failed_) failed_ = "launch_classes";
         return e;
     }
+    if (timing && (e = hipEventRecord(tune_b_, stream_)) != hipSuccess) return e;
     // mirror the device counters
     bool any_done = false;
+    double updates = 0.0;
     for (const Slot &sl : slots_) {
         for (int k = 0; k < sl.G; ++k) {
             const int id = slot_lats_[(size_t)sl.first + (size_t)k];
@@ -477,6 +509,7 @@ hipError_t ResidentEngine::tick(uint32_t n, uint32_t &advanced, KernelTimer *tm)
             const uint32_t total = j.therm_left + j.meas_left;
             if (total == 0) continue;
             const uint32_t swept = std::min(total, sl.n_sweeps);
+            updates += (double)sl.L * (double)sl.L * (double)swept;
             const uint32_t therm = std::min(swept, j.therm_left);
             const uint32_t meas = swept - therm;
             j.t += swept;
@@ -488,6 +521,22 @@ hipError_t ResidentEngine::tick(uint32_t n, uint32_t &advanced, KernelTimer *tm)
     }
     if (any_done) slots_dirty_ = true;
     advanced = chunk;
+    if (timing) {
+        float ms = 0.f;
+        if ((e = hipEventSynchronize(tune_b_)) != hipSuccess) return e;
+        if ((e = hipEventElapsedTime(&ms, tune_a_, tune_b_)) != hipSuccess) return e;
+        tune_[tune_next_].rate = ms > 0.f ? updates / (double)ms : 0.0;
+        if (++tune_next_ == tune_.size()) {
+            size_t best = 1;  // entry 0 only warmed up
+            for (size_t i = 2; i < tune_.size(); ++i)
+                if (tune_[i].rate > tune_[best].rate) best = i;
+            opt_.threads = tune_[best].threads;
+            opt_.min_team = std::min(tune_[best].min_team, tune_[best].threads);
+            opt_.threads_large = tune_[best].threads_large;
+            opt_.max_slot_lds = tune_[best].max_slot_lds;
+            slots_dirty_ = true;
+        }
+    }
     return hipSuccess;
 }
 

@@ -105,6 +105,13 @@ class ResidentEngine {
     hipStream_t stream() const { return stream_; }
     // Name of the HIP call behind the last error returned by this engine.
     const char *failed_call() const { return failed_; }
+    // Autotuning: the next launches each run one candidate configuration
+    // (workgroup size, team bounds, slot LDS cap) and are timed with events;
+    // then the fastest is kept.  Results do not depend on the configuration,
+    // so the jobs advance exactly as they would otherwise.
+    void enable_autotune();
+    bool tuning() const { return tune_next_ < tune_.size(); }
+    const EngineOptions &options() const { return opt_; }
     int group_size(int L, int nl) const;
     // Workgroup size of the slots of side L.
     int threads_for(int L) const
@@ -124,6 +131,14 @@ class ResidentEngine {
         // host mirror of the device counters (exact: chunks advance deterministically)
         uint32_t t = 0, n_measured = 0, therm_left = 0, meas_left = 0;
     };
+    struct TuneCandidate {
+        int threads, min_team, threads_large;
+        uint32_t max_slot_lds;
+        double rate;
+    };
+    std::vector<TuneCandidate> tune_;
+    size_t tune_next_ = 0;
+    hipEvent_t tune_a_ = nullptr, tune_b_ = nullptr;
     void rebuild_slots(uint32_t chunk);
     uint32_t slot_sweeps(int L, int G, uint32_t chunk) const;
     hipError_t launch_classes(KernelTimer *tm);

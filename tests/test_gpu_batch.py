@@ -167,3 +167,18 @@ def test_swendsen_wang_jobs_in_batch(amd, ctx, orc):
                                          job["n_levels"], job["sweeps_per_bin"], sampler=job["sampler"])
             assert np.array_equal(b.spins(j), s_ref), job
             assert np.array_equal(b.records(j), r_ref), job
+
+
+def test_autotune_keeps_results_and_picks_a_candidate(amd, ctx, orc):
+    """The autotuner runs its candidate configurations on the first launches of
+    the real jobs; every job still equals the oracle, and it settles on one of
+    the candidates."""
+    jobs = _jobs(31, 40, Ls=(8, 16, 32, 64, 128))
+    jobs = [dict(j, n_meas=j["n_meas"] + 20) for j in jobs]  # long enough for all candidates
+    with amd.Batch(ctx, jobs, sweeps_per_launch=2, autotune=True) as b:
+        assert b.options()["autotune"] == 1
+        b.run_all()
+        b.sync()
+        opts = b.options()
+        assert opts["autotune"] == 0 and opts["threads"] in (128, 256, 512)
+        _check_vs_oracle(orc, b, jobs)
