@@ -329,7 +329,6 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
     uint32_t *coins = lds + lay.coin;
     uint32_t *cnt = lds + lay.cnt;
     int64_t *rec = reinterpret_cast<int64_t *>(lds + lay.rec);
-    int32_t *sv = reinterpret_cast<int32_t *>(lds + lay.sv);
     uint32_t *pairs = lds + lay.pairs;
 
     // ---- load: meta, spins, zero partial records
@@ -375,49 +374,64 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
     block_sync();
 
     for (uint32_t it = 0; it < n_chunk; ++it) {
+        // Whether any lattice still runs / measures this sweep.  A block-wide
+        // team (one or two lattices per workgroup) evaluates it for every
+        // lattice of the slot in every thread, so the answer is uniform without
+        // a barrier; wave-level teams combine their lanes with a ballot.
         bool running = false, measuring = false;
-        for (uint32_t g = tm.g0; g < G; g += tm.gs) {
-            running = running || in_run(meta + g * kMetaWords, it);
-            measuring = measuring || in_meas(meta + g * kMetaWords, it);
+        if (tm.block) {
+            for (uint32_t g = 0; g < G; ++g) {
+                running = running || in_run(meta + g * kMetaWords, it);
+                measuring = measuring || in_meas(meta + g * kMetaWords, it);
+            }
+        } else {
+            for (uint32_t g = tm.g0; g < G; g += tm.gs) {
+                running = running || in_run(meta + g * kMetaWords, it);
+                measuring = measuring || in_meas(meta + g * kMetaWords, it);
+            }
+            running = team_any(tm, running);
+            measuring = team_any(tm, measuring);
         }
-        if (!team_any(tm, running)) break;
+        if (!running) break;
 
-        // ---- sweep: colour 0, then colour 1
+        // ---- sweep: colour 0, then colour 1; the tie coins of this sample's
+        //      blockings are drawn in the colour-1 phase (they do not depend on
+        //      the spins): call c of the flattened list gives coin words
+        //      4q..4q+3 of its level
         for (uint32_t c = 0; c < 2u; ++c) {
             if (T >= 64u)
                 sweep_colour<true>(meta, spins, L, W, G, tm, it, c);
             else
                 sweep_colour<false>(meta, spins, L, W, G, tm, it, c);
-            team_sync(tm);
-        }
-        if (!team_any(tm, measuring)) continue;
-
-        // ---- coins of every blocking level: call c of the flattened list gives
-        //      coin words 4q..4q+3 of its level
-        for (uint32_t g = tm.g0; g < G; g += tm.gs) {
-            const uint32_t *m = meta + g * kMetaWords;
-            if (!in_meas(m, it)) continue;
-            const uint32_t t = m[kMetaT] + it;
-            for (uint32_t u = tm.u0; u < T; u += tm.us) {
-                for (uint32_t c = u; c < lay.coin_calls; c += T) {
-                    uint32_t l = 0, q = c, off = 0;
-                    for (; l + 1u < nl; ++l) {
-                        const uint32_t calls = (level_words(L, l + 1u) + 3u) / 4u;
-                        if (q < calls) break;
-                        q -= calls;
-                        off += level_words(L, l + 1u);
+            if (c == 1u && measuring) {
+                for (uint32_t g = tm.g0; g < G; g += tm.gs) {
+                    const uint32_t *m = meta + g * kMetaWords;
+                    if (!in_meas(m, it)) continue;
+                    const uint32_t t = m[kMetaT] + it;
+                    for (uint32_t u = tm.u0; u < T; u += tm.us) {
+                        for (uint32_t cc = u; cc < lay.coin_calls; cc += T) {
+                            uint32_t l = 0, q = cc, off = 0;
+                            for (; l + 1u < nl; ++l) {
+                                const uint32_t calls = (level_words(L, l + 1u) + 3u) / 4u;
+                                if (q < calls) break;
+                                q -= calls;
+                                off += level_words(L, l + 1u);
+                            }
+                            const uint32_t n_words = level_words(L, l + 1u);
+                            const u32x4 r =
+                                philox(q, t, 0u, (kTagTie << 28) | (l << 16), m[kMetaK0], m[kMetaK1]);
+                            uint32_t *cw = coins + g * lay.lvl_words + off + 4u * q;
+                            cw[0] = r.v0;
+                            if (4u * q + 1u < n_words) cw[1] = r.v1;
+                            if (4u * q + 2u < n_words) cw[2] = r.v2;
+                            if (4u * q + 3u < n_words) cw[3] = r.v3;
+                        }
                     }
-                    const uint32_t n_words = level_words(L, l + 1u);
-                    const u32x4 r = philox(q, t, 0u, (kTagTie << 28) | (l << 16), m[kMetaK0], m[kMetaK1]);
-                    uint32_t *cw = coins + g * lay.lvl_words + off + 4u * q;
-                    cw[0] = r.v0;
-                    if (4u * q + 1u < n_words) cw[1] = r.v1;
-                    if (4u * q + 2u < n_words) cw[2] = r.v2;
-                    if (4u * q + 3u < n_words) cw[3] = r.v3;
                 }
             }
+            team_sync(tm);
         }
-        team_sync(tm);
+        if (!measuring) continue;
 
         // ---- level 0: operator counts and blocking into level 1
         for (uint32_t g = tm.g0; g < G; g += tm.gs) {
@@ -498,22 +512,17 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
         }
 #endif
 
-        // ---- operator values, then the sample's sums and products into the
-        //      partial records
-        for (uint32_t g = tm.g0; g < G; g += tm.gs) {
-            if (!in_meas(meta + g * kMetaWords, it)) continue;
-            for (uint32_t u = tm.u0; u < T; u += tm.us)
-                for (uint32_t i = u; i < lay.n_sv; i += T)
-                    sv[g * lay.n_sv + i] = (int32_t)sv_value(cnt + g * cstride, nw, nl, L, i);
-        }
-        team_sync(tm);
+        // ---- the sample's sums and products into the partial records; each
+        //      entry forms its two operator values from the counts.  No barrier
+        //      follows: the next writes of the counts (level 0 of the next
+        //      sample) come after the two sweep barriers.
         for (uint32_t g = tm.g0; g < G; g += tm.gs) {
             const uint32_t *m = meta + g * kMetaWords;
             if (!in_meas(m, it)) continue;
             const uint32_t q = m[kMetaNMeas] + (it - m[kMetaTherm]);
             const uint32_t spb = m[kMetaSpb];
             const bool new_bin = q % spb == 0u && q > m[kMetaNMeas];
-            const int32_t *svg = sv + g * lay.n_sv;
+            const uint32_t *cntg = cnt + g * cstride;
             for (uint32_t u = tm.u0; u < T; u += tm.us) {
                 for (uint32_t k = u; k < rs; k += T) {
                     int64_t &r = rec[g * rs + k];
@@ -522,11 +531,10 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
                         r = 0;
                     }
                     const uint32_t xy = pairs[k];
-                    r += (int64_t)svg[xy & 0xFFFFu] * (int64_t)svg[xy >> 16];
+                    r += sv_value(cntg, nw, nl, L, xy & 0xFFFFu) * sv_value(cntg, nw, nl, L, xy >> 16);
                 }
             }
         }
-        team_sync(tm);
 #ifdef MCRG_HOST_EMULATION
         for (uint32_t i = 0; i < G * cstride; ++i) cnt[i] = 0u;
 #endif
