@@ -62,3 +62,28 @@ def test_nu_metropolis_batched_matches_published_n16(amd, ctx):
         print(f"level {lvl}: nu = {r['nu']:.5f} +- {r['nu_err']:.5f} (published {pub:.5f})", flush=True)
         assert 0 < r["nu_err"] < 0.004
         assert abs(r["nu"] - pub) < 5 * r["nu_err"] + 0.0005, (lvl, r["nu"], pub, r["nu_err"])
+
+
+def test_config_l8_plain_yt_matches_oracle(amd, ctx, orc):
+    """The plain single-lattice case: L = 8, |K| = 0.4406868, seed 1, 1e4
+    thermalisation and 1e5 measured sweeps, one 8 -> 4 blocking, on the
+    device's HBM schedule.  Records equal the oracle's bit for bit, so y_t and
+    its jackknife error match exactly; y_t is also within its error of the
+    value of an independent seed."""
+    L, J, nl, spb = 8, 0.4406868, 1, 2000
+    print("config L=8 plain", flush=True)
+    with amd.Chain(ctx, L, J, 1, n_levels=nl, sweeps_per_bin=spb, max_bins=50) as ch:
+        ch.sweep(10000)
+        ch.sweep(100000, measure=True)
+        rec = ch.records()
+    _, rec_ref = orc.run_chain(L, J, 1, 10000, 100000, nl, spb)
+    assert np.array_equal(rec, rec_ref)
+    for n_ops in (2, 3):
+        r = amd.rg_exponent(rec, nl, 0, n_ops)
+        m, e = orc.jackknife_yt(rec_ref, nl, 0, n_ops)
+        assert r["yt"] == pytest.approx(m, rel=1e-12) and r["yt_err"] == pytest.approx(e, rel=1e-9)
+        print(f"n_ops={n_ops}: y_t = {r['yt']:.5f} +- {r['yt_err']:.5f}", flush=True)
+    _, rec2 = orc.run_chain(L, J, 2, 10000, 100000, nl, spb)
+    r2 = amd.rg_exponent(rec2, nl, 0, 2)
+    r1 = amd.rg_exponent(rec, nl, 0, 2)
+    assert abs(r1["yt"] - r2["yt"]) < 4 * math.hypot(r1["yt_err"], r2["yt_err"])
