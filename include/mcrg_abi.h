@@ -167,8 +167,7 @@ mcrg_status mcrg_chain_operators(mcrg_chain ch, int64_t out[4]);
 typedef struct mcrg_batch_s *mcrg_batch;
 
 typedef struct {
-    int32_t L;              /* power of two, 2 <= L <= 16384 (server: 8..512);
-                               two or more blockings need L <= 2048            */
+    int32_t L;              /* power of two, 2 <= L <= 16384 (server: 8..512)  */
     double coupling;        /* J = |K| >= 0                                    */
     uint64_t seed;          /* Philox key                                      */
     int32_t n_therm;        /* unmeasured sweeps                               */

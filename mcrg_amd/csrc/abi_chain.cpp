@@ -204,8 +204,6 @@ mcrg_status mcrg_chain_create(mcrg_ctx ctx, const mcrg_chain_desc *desc, mcrg_ch
     MCRG_REQUIRE(std::isfinite(d.coupling) && d.coupling >= 0.0, "coupling J = |K| must be >= 0");
     MCRG_REQUIRE(d.n_levels >= 0 && d.n_levels <= ilog2(d.L) - 1 && d.n_levels <= (int)kMaxLevels,
                  "n_levels must be in [0, log2(L) - 1]");
-    MCRG_REQUIRE(d.n_levels <= 1 || d.L <= kMaxHbmMeasureL,
-                 "two or more blockings on the HBM schedule need L <= 2048");
     MCRG_REQUIRE(d.sweeps_per_bin >= 1 && d.max_bins >= 1, "sweeps_per_bin and max_bins must be >= 1");
     MCRG_REQUIRE(d.layout == MCRG_LAYOUT_INT8, "only MCRG_LAYOUT_INT8 chains run on this schedule");
     MCRG_REQUIRE(d.sampler == MCRG_SAMPLER_METROPOLIS || d.sampler == MCRG_SAMPLER_SWENDSEN_WANG,
@@ -292,7 +290,7 @@ mcrg_status mcrg_chain_sweep(mcrg_chain ch, int32_t n_sweeps, int32_t measure)
     MCRG_REQUIRE((uint64_t)ch->t + (uint64_t)n_sweeps <= 0xFFFFFFFFull, "sweep counter overflow");
     MCRG_HIP(hipSetDevice(ch->ctx->device));
     KernelTimer *tm = ch->ctx->timer.enabled ? &ch->ctx->timer : nullptr;
-    MCRG_HIP(ch->runner.run(ch->d_desc, 1, d.L, d.sampler, (uint32_t)n_sweeps, measure != 0, ch->stream, tm));
+    MCRG_HIP(ch->runner.run(ch->d_desc, 1, d.L, d.n_levels, d.sampler, (uint32_t)n_sweeps, measure != 0, ch->stream, tm));
     ch->t += (uint32_t)n_sweeps;
     if (measure) ch->n_measured += (uint32_t)n_sweeps;
     return MCRG_OK;
@@ -369,11 +367,11 @@ mcrg_status mcrg_chain_operators(mcrg_chain ch, int64_t out[4])
     MCRG_REQUIRE(ch != nullptr && out != nullptr, "null argument");
     MCRG_HIP(hipSetDevice(ch->ctx->device));
     hipStream_t s = ch->stream;
-    MCRG_HIP(hipMemsetAsync(ch->host.sums0, 0, 4 * sizeof(unsigned long long), s));
+    MCRG_HIP(hipMemsetAsync(ch->host.sums, 0, 4 * sizeof(unsigned long long), s));
     MCRG_HIP(launch_int8_measure0(ch->d_desc, 1, ch->desc.L, 0u, s));
     unsigned long long cnt[4];
-    MCRG_HIP(hipMemcpyAsync(cnt, ch->host.sums0, sizeof(cnt), hipMemcpyDeviceToHost, s));
-    MCRG_HIP(hipMemsetAsync(ch->host.sums0, 0, 4 * sizeof(unsigned long long), s));
+    MCRG_HIP(hipMemcpyAsync(cnt, ch->host.sums, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    MCRG_HIP(hipMemsetAsync(ch->host.sums, 0, 4 * sizeof(unsigned long long), s));
     MCRG_HIP(hipStreamSynchronize(s));
     const int64_t n0 = (int64_t)ch->desc.L * ch->desc.L;
     out[0] = 2 * n0 - 2 * (int64_t)cnt[0];

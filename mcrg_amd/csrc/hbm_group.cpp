@@ -12,11 +12,13 @@ HbmBlock hbm_block(int L, int n_levels, int max_bins, int sampler)
 {
     HbmBlock b;
     const size_t n_spins = (size_t)L * (size_t)L;
-    const size_t n_lvl1 = ((size_t)(L / 2) * (size_t)(L / 2) + 31u) / 32u;
+    // levels 1 .. max(n_levels, 1): level 1 is always written by the level-0 measurement
+    const uint32_t top = n_levels > 1 ? (uint32_t)n_levels : 1u;
+    const size_t n_lvl = L >= 2 ? level_offset((uint32_t)L, top + 1u) : 0u;
     b.spins = 0;
-    b.lvl1 = align256(b.spins + n_spins);
-    b.sums = align256(b.lvl1 + n_lvl1 * 4u);
-    b.state = align256(b.sums + 4u * 8u);
+    b.lvl = align256(b.spins + n_spins);
+    b.sums = align256(b.lvl + n_lvl * 4u);
+    b.state = align256(b.sums + kSumWords * 8u);
     b.rec = align256(b.state + 2u * 4u);
     b.labels = align256(b.rec + (size_t)max_bins * record_size((uint32_t)n_levels) * 8u);
     b.bytes = align256(b.labels + (sampler == MCRG_SAMPLER_SWENDSEN_WANG ? n_spins * 4u : 0u));
@@ -29,8 +31,8 @@ HbmLattice hbm_descriptor(char *base, const HbmBlock &b, int L, int n_levels, in
     HbmLattice h;
     std::memset(&h, 0, sizeof(h));
     h.spins = reinterpret_cast<int8_t *>(base + b.spins);
-    h.lvl1 = reinterpret_cast<uint32_t *>(base + b.lvl1);
-    h.sums0 = reinterpret_cast<unsigned long long *>(base + b.sums);
+    h.lvl = reinterpret_cast<uint32_t *>(base + b.lvl);
+    h.sums = reinterpret_cast<unsigned long long *>(base + b.sums);
     h.state = reinterpret_cast<uint32_t *>(base + b.state);
     h.records = reinterpret_cast<int64_t *>(base + b.rec);
     h.k0 = (uint32_t)seed;
@@ -45,7 +47,7 @@ HbmLattice hbm_descriptor(char *base, const HbmBlock &b, int L, int n_levels, in
     return h;
 }
 
-hipError_t enqueue_sweeps(const HbmLattice *d, int n_lats, int L, int sampler, uint32_t n, bool measure,
+hipError_t enqueue_sweeps(const HbmLattice *d, int n_lats, int L, int n_levels, int sampler, uint32_t n, bool measure,
                           hipStream_t s, KernelTimer *tm)
 {
     hipError_t e;
@@ -66,7 +68,7 @@ hipError_t enqueue_sweeps(const HbmLattice *d, int n_lats, int L, int sampler, u
             if ((e = launch_int8_measure0(d, n_lats, L, i, s)) != hipSuccess) return e;
             if (tm) tm->end(ev, s);
             ev = tm ? tm->begin(kKindLevels, s) : -1;
-            if ((e = launch_levels_accumulate(d, n_lats, i, i, s)) != hipSuccess) return e;
+            if ((e = launch_levels_accumulate(d, n_lats, L, n_levels, i, i, s)) != hipSuccess) return e;
             if (tm) tm->end(ev, s);
         }
     }
@@ -79,7 +81,7 @@ SweepRunner::~SweepRunner()
         if (g) (void)hipGraphExecDestroy(g);
 }
 
-hipError_t SweepRunner::run(const HbmLattice *d, int n_lats, int L, int sampler, uint32_t n, bool measure,
+hipError_t SweepRunner::run(const HbmLattice *d, int n_lats, int L, int n_levels, int sampler, uint32_t n, bool measure,
                             hipStream_t s, KernelTimer *tm)
 {
     hipError_t e;
@@ -92,7 +94,7 @@ hipError_t SweepRunner::run(const HbmLattice *d, int n_lats, int L, int sampler,
             // so every replay continues where the previous one stopped
             hipGraph_t g = nullptr;
             if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
-            hipError_t ce = enqueue_sweeps(d, n_lats, L, sampler, kGraphSweeps, measure, s, nullptr);
+            hipError_t ce = enqueue_sweeps(d, n_lats, L, n_levels, sampler, kGraphSweeps, measure, s, nullptr);
             if (ce == hipSuccess) ce = launch_advance(d, n_lats, kGraphSweeps, measure ? kGraphSweeps : 0u, s);
             e = hipStreamEndCapture(s, &g);
             if (ce != hipSuccess) {
@@ -112,7 +114,7 @@ hipError_t SweepRunner::run(const HbmLattice *d, int n_lats, int L, int sampler,
     }
     const uint32_t rest = n - done;
     if (rest == 0) return hipSuccess;
-    if ((e = enqueue_sweeps(d, n_lats, L, sampler, rest, measure, s, tm)) != hipSuccess) return e;
+    if ((e = enqueue_sweeps(d, n_lats, L, n_levels, sampler, rest, measure, s, tm)) != hipSuccess) return e;
     return launch_advance(d, n_lats, rest, measure ? rest : 0u, s);
 }
 
@@ -134,7 +136,6 @@ std::string HbmGroup::check(const JobSpec &j)
     if (!(j.coupling >= 0.0) || !std::isfinite(j.coupling)) return "coupling J = |K| must be >= 0";
     if (j.n_levels < 0 || j.n_levels > ilog2(j.L) - 1 || j.n_levels > (int)kMaxLevels)
         return "n_levels must be in [0, log2(L) - 1]";
-    if (j.n_levels > 1 && j.L > kMaxHbmMeasureL) return "two or more blockings on the HBM schedule need L <= 2048";
     if (j.sweeps_per_bin < 1) return "sweeps_per_bin must be >= 1";
     if ((uint64_t)j.n_therm + (uint64_t)j.n_meas > 0xFFFFFFFFull) return "too many sweeps";
     if (j.sampler != MCRG_SAMPLER_METROPOLIS && j.sampler != MCRG_SAMPLER_SWENDSEN_WANG)
@@ -175,13 +176,13 @@ hipError_t HbmGroup::advance(uint32_t n, hipStream_t s, KernelTimer *tm)
     hipError_t e;
     const uint32_t therm = std::min(n, therm_left_);
     if (therm > 0) {
-        if ((e = runner_.run(d_desc_, size(), spec().L, spec().sampler, therm, false, s, tm)) != hipSuccess) return e;
+        if ((e = runner_.run(d_desc_, size(), spec().L, spec().n_levels, spec().sampler, therm, false, s, tm)) != hipSuccess) return e;
         therm_left_ -= therm;
         t_ += therm;
     }
     const uint32_t meas = std::min(n - therm, meas_left_);
     if (meas > 0) {
-        if ((e = runner_.run(d_desc_, size(), spec().L, spec().sampler, meas, true, s, tm)) != hipSuccess) return e;
+        if ((e = runner_.run(d_desc_, size(), spec().L, spec().n_levels, spec().sampler, meas, true, s, tm)) != hipSuccess) return e;
         meas_left_ -= meas;
         t_ += meas;
         q_ += meas;

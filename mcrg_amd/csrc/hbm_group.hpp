@@ -16,7 +16,7 @@ namespace mcrg {
 
 // Byte offsets (256-aligned) of one lattice's buffers inside its device block.
 struct HbmBlock {
-    size_t spins = 0, lvl1 = 0, sums = 0, state = 0, rec = 0, labels = 0, bytes = 0;
+    size_t spins = 0, lvl = 0, sums = 0, state = 0, rec = 0, labels = 0, bytes = 0;
 };
 HbmBlock hbm_block(int L, int n_levels, int max_bins, int sampler);
 // Descriptor of the lattice whose block starts at `base`.
@@ -25,7 +25,7 @@ HbmLattice hbm_descriptor(char *base, const HbmBlock &b, int L, int n_levels, in
 
 // Launches n sweeps (t_off 0..n-1) of n_lats lattices of side L without
 // advancing their counters.
-hipError_t enqueue_sweeps(const HbmLattice *d, int n_lats, int L, int sampler, uint32_t n, bool measure,
+hipError_t enqueue_sweeps(const HbmLattice *d, int n_lats, int L, int n_levels, int sampler, uint32_t n, bool measure,
                           hipStream_t s, KernelTimer *tm);
 
 // Runs of sweeps followed by the counter advance.  Whole blocks of
@@ -38,7 +38,7 @@ class SweepRunner {
     SweepRunner(const SweepRunner &) = delete;
     SweepRunner &operator=(const SweepRunner &) = delete;
     ~SweepRunner();
-    hipError_t run(const HbmLattice *d, int n_lats, int L, int sampler, uint32_t n, bool measure, hipStream_t s,
+    hipError_t run(const HbmLattice *d, int n_lats, int L, int n_levels, int sampler, uint32_t n, bool measure, hipStream_t s,
                    KernelTimer *tm);
 
   private:

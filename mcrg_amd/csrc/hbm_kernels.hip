@@ -12,8 +12,10 @@
 
 //   k_int8_measure0       level-0 NN/NNN/P + M and b=2 majority blocking into a
 //                         bit-packed level-1 lattice (src/lattice.cpp:102-120, src/mcrg.cpp:314-348)
-//   k_levels_accumulate   levels 1..nl in one workgroup per lattice, then the
-//                         per-sample sums and products into the bin record
+//   k_rm_level            one large blocked level: operators and the next
+//                         blocking, over a grid (src/mcrg.cpp:80-86, src/lattice.cpp:84-120)
+//   k_levels_accumulate   the small levels in one workgroup per lattice, then
+//                         the per-sample sums and products into the bin record
 //                         (src/mcrg.cpp:72-98)
 #include "device_common.hpp"
 #include "hbm_kernels.hpp"
@@ -290,19 +292,25 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
-// Adds four per-thread counts into sums[0..3] with one atomic per wave.
-__device__ __forceinline__ void add4(unsigned long long *sums, uint32_t a, uint32_t b, uint32_t c,
-                                     uint32_t d)
+// Adds the N per-thread counts v[0..N-1] of a 256-thread workgroup into
+// dst[0..N-1]: wave sums, then one atomic per counter per workgroup.  Every
+// thread of the workgroup calls it.
+template <int N>
+__device__ __forceinline__ void block_add(unsigned long long *dst, uint32_t (&v)[N])
 {
-    a = wave_sum(a);
-    b = wave_sum(b);
-    c = wave_sum(c);
-    d = wave_sum(d);
+    __shared__ uint32_t part[N][4];
+    const uint32_t wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
     if ((threadIdx.x & 63u) == 0u) {
-        atomicAdd(sums + 0, (unsigned long long)a);
-        atomicAdd(sums + 1, (unsigned long long)b);
-        atomicAdd(sums + 2, (unsigned long long)c);
-        atomicAdd(sums + 3, (unsigned long long)d);
+#pragma unroll
+        for (int k = 0; k < N; ++k) part[k][wave] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)N) {
+        const uint32_t k = threadIdx.x;
+        const uint32_t tot = part[k][0] + part[k][1] + part[k][2] + part[k][3];
+        if (tot) atomicAdd(dst + k, (unsigned long long)tot);
     }
 }
 
@@ -311,18 +319,19 @@ __device__ __forceinline__ void add4(unsigned long long *sums, uint32_t a, uint3
 // bonds, dNNN likewise over the 2L^2 diagonal bonds, dP = #{plaquettes with an
 // odd number of -1 spins}, ndown = #{-1 spins}.  Then NN = 2L^2 - 2 dNN,
 // NNN = 2L^2 - 2 dNNN, P = L^2 - 2 dP, M = L^2 - 2 ndown.
-__device__ __forceinline__ void row_bits64(const int8_t *p, uint32_t &lo, uint32_t &hi)
+// Sign bits of 32 int8 spins (bit i = spin x0 + i is -1).
+__device__ __forceinline__ uint32_t row_bits32(const int8_t *p)
 {
-    uint32_t d[16];
-    load64(p, d);
-    lo = 0;
-    hi = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) lo |= sign4(d[i]) << (4 * i);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) hi |= sign4(d[8 + i]) << (4 * i);
+    const uint4 *v = reinterpret_cast<const uint4 *>(p);
+    const uint4 a = v[0], b = v[1];
+    return sign4(a.x) | (sign4(a.y) << 4) | (sign4(a.z) << 8) | (sign4(a.w) << 12) | (sign4(b.x) << 16) |
+           (sign4(b.y) << 20) | (sign4(b.z) << 24) | (sign4(b.w) << 28);
 }
 
+// One item per (block row by, 32-column chunk q), L >= 32: the operators of
+// the sites of rows 2by and 2by + 1 in the chunk and the 16 blocked spins of
+// the chunk, stored as half-word by * (L/32) + q of level 1 (bit offset
+// 16 (by * L/32 + q) = by * L/2 + 16 q, as in row-major bits).
 __global__ __launch_bounds__(256) void k_int8_measure0(const HbmLattice *__restrict__ lats,
                                                        uint32_t t_off)
 {
@@ -331,39 +340,36 @@ __global__ __launch_bounds__(256) void k_int8_measure0(const HbmLattice *__restr
     const uint32_t t = lat.state[0] + t_off;
     const int8_t *spins = lat.spins;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t dnn = 0, dnnn = 0, dp = 0, ndown = 0;
-    if (L >= 64) {
-        const uint32_t wpr = L >> 6;  // 64-site chunks per row == level-1 words per row
-        const uint32_t n_items = (L >> 1) * wpr;
+    uint32_t c[4] = {0u, 0u, 0u, 0u};  // dNN, dNNN, dP, ndown
+    if (L >= 32) {
+        const uint32_t cpr = L >> 5;  // 32-site chunks per row == level-1 half-words per block row
+        const uint32_t n_items = (L >> 1) * cpr;
         if (i < n_items) {
-            const uint32_t by = i >> __builtin_ctz(wpr), q = i & (wpr - 1u);
-            uint32_t lo[3], hi[3], nx[3];
+            const uint32_t by = i >> __builtin_ctz(cpr), q = i & (cpr - 1u);
+            uint32_t a[3], nx[3];
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
                 const uint32_t y = (2u * by + (uint32_t)r) & (L - 1u);
                 const int8_t *rowp = spins + (size_t)y * L;
-                row_bits64(rowp + 64u * q, lo[r], hi[r]);
-                nx[r] = sbit(rowp[(64u * q + 64u) & (L - 1u)]);
+                a[r] = row_bits32(rowp + 32u * q);
+                nx[r] = sbit(rowp[(32u * q + 32u) & (L - 1u)]);
             }
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
-                // right = s(x+1): shift the 64-bit row right by one, bring in the next chunk's bit.
-                const uint32_t alo = lo[r], ahi = hi[r], dlo = lo[r + 1], dhi = hi[r + 1];
-                const uint32_t rlo = (alo >> 1) | (ahi << 31), rhi = (ahi >> 1) | (nx[r] << 31);
-                const uint32_t glo = (dlo >> 1) | (dhi << 31), ghi = (dhi >> 1) | (nx[r + 1] << 31);
-                dnn += popc(alo ^ rlo) + popc(ahi ^ rhi) + popc(alo ^ dlo) + popc(ahi ^ dhi);
-                dnnn += popc(alo ^ glo) + popc(ahi ^ ghi) + popc(rlo ^ dlo) + popc(rhi ^ dhi);
-                dp += popc(alo ^ rlo ^ dlo ^ glo) + popc(ahi ^ rhi ^ dhi ^ ghi);
-                ndown += popc(alo) + popc(ahi);
+                // right = s(x+1): the row shifted right by one with the next chunk's first bit.
+                const uint32_t s = a[r], d = a[r + 1];
+                const uint32_t rt = (s >> 1) | (nx[r] << 31), g = (d >> 1) | (nx[r + 1] << 31);
+                c[0] += popc(s ^ rt) + popc(s ^ d);
+                c[1] += popc(s ^ g) + popc(rt ^ d);
+                c[2] += popc(s ^ rt ^ d ^ g);
+                c[3] += popc(s);
             }
-            // b = 2 blocks: even/odd columns of rows 2by and 2by+1.
-            const uint32_t coin = tie_coin_word(i, t, 0u, lat.k0, lat.k1);
-            uint32_t out = 0;
-            out = block16(lo[0], lo[1], coin & 0xFFFFu) | (block16(hi[0], hi[1], coin >> 16) << 16);
-            lat.lvl1[i] = out;
+            // coin word i >> 1 of level 0, its half i & 1 (the half-word's bit offset mod 32)
+            const uint32_t coin = tie_coin_word(i >> 1, t, 0u, lat.k0, lat.k1) >> (16u * (i & 1u));
+            reinterpret_cast<uint16_t *>(lat.lvl)[i] = (uint16_t)block16(a[0], a[1], coin & 0xFFFFu);
         }
     } else {
-        // L < 64: one thread per level-1 word, site by site.
+        // L < 32: one thread per level-1 word, site by site.
         const uint32_t Lb = L >> 1;
         const uint32_t nb = Lb * Lb;
         const uint32_t n_words = (nb + 31u) >> 5;
@@ -380,33 +386,59 @@ __global__ __launch_bounds__(256) void k_int8_measure0(const HbmLattice *__restr
                         const uint32_t x = 2u * bx + u, y = 2u * by + v;
                         const uint32_t xp = (x + 1u) & (L - 1u), yp = (y + 1u) & (L - 1u);
                         const uint32_t a = sbit(spins[y * L + x]), b = sbit(spins[y * L + xp]);
-                        const uint32_t c = sbit(spins[yp * L + x]), d = sbit(spins[yp * L + xp]);
-                        dnn += (a ^ b) + (a ^ c);
-                        dnnn += (a ^ d) + (b ^ c);
-                        dp += a ^ b ^ c ^ d;
-                        ndown += a;
+                        const uint32_t cc = sbit(spins[yp * L + x]), d = sbit(spins[yp * L + xp]);
+                        c[0] += (a ^ b) + (a ^ cc);
+                        c[1] += (a ^ d) + (b ^ cc);
+                        c[2] += a ^ b ^ cc ^ d;
+                        c[3] += a;
                         nd += a;
                     }
                 }
                 const uint32_t bit = nd >= 3u ? 1u : nd <= 1u ? 0u : ((coin >> j) & 1u);
                 out |= bit << j;
             }
-            lat.lvl1[i] = out;
+            lat.lvl[i] = out;
         }
     }
-    add4(lat.sums0, dnn, dnnn, dp, ndown);
+    block_add<4>(lat.sums, c);
 }
 
 // ---------------------------------------------------------------- levels >= 1
+// A grid-measured level l (side Ll >= kGridLevelMinL): one thread per word of
+// the level for its operators, and one per word of level l + 1 for the
+// blocking when l < nl (ties take the coin words of level l).
+__global__ __launch_bounds__(256) void k_rm_level(const HbmLattice *__restrict__ lats, uint32_t level,
+                                                  uint32_t t_off)
+{
+    const HbmLattice &lat = lats[blockIdx.y];
+    const uint32_t L = (uint32_t)lat.L;
+    const uint32_t t = lat.state[0] + t_off;
+    const uint32_t Ll = L >> level;
+    const uint32_t *src = lat.lvl + level_offset(L, level);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c[3] = {0u, 0u, 0u};
+    if (i < rm_ops_items(Ll)) rm_ops_item(src, Ll, i, c[0], c[1], c[2]);
+    if (level < (uint32_t)lat.n_levels && i < rm_block_items(Ll))
+        lat.lvl[level_offset(L, level + 1u) + i] = rm_block_item(src, Ll, i, tie_coin_word(i, t, level, lat.k0, lat.k1));
+    block_add<3>(lat.sums + 4u + 3u * (level - 1u), c);
+}
+
+constexpr uint32_t kLevelsThreads = 512;
+
 // Disagreement counts of one RM level, summed over the workgroup into cnt[0..2].
 __device__ void rm_ops(const uint32_t *buf, uint32_t Ll, uint32_t *cnt)
 {
     uint32_t dnn = 0, dnnn = 0, dp = 0;
     const uint32_t n = rm_ops_items(Ll);
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) rm_ops_item(buf, Ll, i, dnn, dnnn, dp);
-    atomicAdd(cnt + 0, dnn);
-    atomicAdd(cnt + 1, dnnn);
-    atomicAdd(cnt + 2, dp);
+    dnn = wave_sum(dnn);
+    dnnn = wave_sum(dnnn);
+    dp = wave_sum(dp);
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicAdd(cnt + 0, dnn);
+        atomicAdd(cnt + 1, dnnn);
+        atomicAdd(cnt + 2, dp);
+    }
 }
 
 // b = 2 majority blocking of an RM level (size Ll) into dst (size Ll/2).
@@ -418,73 +450,90 @@ __device__ void rm_block(const uint32_t *src, uint32_t Ll, uint32_t *dst, uint32
         dst[i] = rm_block_item(src, Ll, i, tie_coin_word(i, t, level, k0, k1));
 }
 
-// One workgroup per lattice.  Levels 1..nl: level 1 is read from global memory
-// (written by k_int8_measure0), deeper levels live in LDS.  Thread 0 then turns
-// the counts into operator values and adds the sample to its bin record.
-__global__ __launch_bounds__(1024) void k_levels_accumulate(const HbmLattice *__restrict__ lats,
-                                                            uint32_t t_off, uint32_t q_off)
+// Entry j of the per-sample contribution to a bin record (layout of off_S,
+// off_self, off_cross, off_nn0, off_M) from the operator values S[l][a] and
+// the level-0 magnetisation M.
+__device__ __forceinline__ int64_t record_term(uint32_t j, uint32_t nl, const int64_t (*S)[3], int64_t M)
+{
+    const uint32_t n_s = 3u * (nl + 1u), n_self = 9u * (nl + 1u), n_cross = 9u * nl;
+    if (j < n_s) return S[j / 3u][j % 3u];
+    j -= n_s;
+    if (j < n_self) {
+        const uint32_t l = j / 9u, a = j % 3u, b = (j % 9u) / 3u;
+        return S[l][a] * S[l][b];
+    }
+    j -= n_self;
+    if (j < n_cross) {
+        const uint32_t l = j / 9u + 1u, a = j % 3u, b = (j % 9u) / 3u;
+        return S[l][a] * S[l - 1u][b];
+    }
+    j -= n_cross;
+    if (j < nl) return S[j + 1u][0] * S[0][0];
+    j -= nl;
+    return j == 0u ? M : j == 1u ? (M < 0 ? -M : M) : j == 2u ? M * M : 1;
+}
+
+// One workgroup per lattice: the levels below kGridLevelMinL (the first read
+// from global memory, the deeper ones in LDS), then the operator values of
+// every level (grid-measured levels and level 0 from the counters, which it
+// clears) and the sample's terms added to its bin record, one entry per thread.
+__global__ __launch_bounds__(kLevelsThreads) void k_levels_accumulate(const HbmLattice *__restrict__ lats,
+                                                                      uint32_t t_off, uint32_t q_off)
 {
     const HbmLattice &lat = lats[blockIdx.x];
     const uint32_t L = (uint32_t)lat.L;
     const uint32_t nl = (uint32_t)lat.n_levels;
     const uint32_t t = lat.state[0] + t_off;
+    const uint32_t l0 = n_grid_levels(L, nl) + 1u;  // first level of this workgroup
     __shared__ uint32_t cnt[kMaxLevels + 1][3];
-    __shared__ uint32_t bufA[kLdsLevelWords];
-    __shared__ uint32_t bufB[kLdsLevelWords / 4];
+    __shared__ uint32_t bufA[level_words(kWgLevelMaxL / 2u)];
+    __shared__ uint32_t bufB[level_words(kWgLevelMaxL / 4u)];
+    __shared__ int64_t S[kMaxLevels + 1][3];
+    __shared__ int64_t Msh;
     for (uint32_t i = threadIdx.x; i < (kMaxLevels + 1) * 3; i += blockDim.x) (&cnt[0][0])[i] = 0u;
     __syncthreads();
-    const uint32_t *cur = lat.lvl1;
-    uint32_t Ll = L >> 1;
-    for (uint32_t l = 1; l <= nl; ++l) {
-        rm_ops(cur, Ll, cnt[l]);
-        if (l < nl) {
-            uint32_t *dst = (l & 1u) ? bufA : bufB;
-            __syncthreads();
-            rm_block(cur, Ll, dst, t, l, lat.k0, lat.k1);
-            __syncthreads();
-            cur = dst;
-            Ll >>= 1;
+    if (l0 <= nl) {
+        const uint32_t *cur = lat.lvl + level_offset(L, l0);
+        uint32_t Ll = L >> l0;
+        for (uint32_t l = l0; l <= nl; ++l) {
+            rm_ops(cur, Ll, cnt[l]);
+            if (l < nl) {
+                uint32_t *dst = ((l - l0) & 1u) ? bufB : bufA;
+                __syncthreads();
+                rm_block(cur, Ll, dst, t, l, lat.k0, lat.k1);
+                __syncthreads();
+                cur = dst;
+                Ll >>= 1;
+            }
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int64_t S[kMaxLevels + 1][3];
-        unsigned long long *s0 = lat.sums0;
-        const int64_t n0 = (int64_t)L * L;
-        S[0][0] = 2 * n0 - 2 * (int64_t)s0[0];
-        S[0][1] = 2 * n0 - 2 * (int64_t)s0[1];
-        S[0][2] = n0 - 2 * (int64_t)s0[2];
-        const int64_t M = n0 - 2 * (int64_t)s0[3];
-        s0[0] = 0ull;
-        s0[1] = 0ull;
-        s0[2] = 0ull;
-        s0[3] = 0ull;
-        for (uint32_t l = 1; l <= nl; ++l) {
-            const int64_t nlv = (int64_t)(L >> l) * (L >> l);
-            S[l][0] = 2 * nlv - 2 * (int64_t)cnt[l][0];
-            S[l][1] = 2 * nlv - 2 * (int64_t)cnt[l][1];
-            S[l][2] = nlv - 2 * (int64_t)cnt[l][2];
+    // operator values; every counter is read and cleared by one thread
+    const uint32_t n_s = 3u * (nl + 1u);
+    for (uint32_t i = threadIdx.x; i <= n_s; i += blockDim.x) {
+        if (i == n_s) {
+            Msh = (int64_t)L * L - 2 * (int64_t)lat.sums[3];
+            lat.sums[3] = 0ull;
+            continue;
         }
-        const uint32_t q = lat.state[1] + q_off;
-        const uint32_t bin = q / (uint32_t)lat.sweeps_per_bin;
-        const uint32_t rs = record_size(nl);
-        int64_t *rec = lat.records + (size_t)bin * rs;
-        for (uint32_t l = 0; l <= nl; ++l) {
-            for (uint32_t a = 0; a < 3u; ++a) {
-                rec[off_S(l, a)] += S[l][a];
-                for (uint32_t b = 0; b < 3u; ++b) {
-                    rec[off_self(nl, l, a, b)] += S[l][a] * S[l][b];
-                    if (l >= 1u) rec[off_cross(nl, l, a, b)] += S[l][a] * S[l - 1][b];
-                }
-            }
-            if (l >= 1u) rec[off_nn0(nl, l)] += S[l][0] * S[0][0];
+        const uint32_t l = i / 3u, a = i % 3u;
+        const int64_t nlv = (int64_t)(L >> l) * (L >> l);
+        int64_t d;
+        if (l == 0u || l < l0) {
+            unsigned long long *sp = l == 0u ? lat.sums + a : lat.sums + 4u + 3u * (l - 1u) + a;
+            d = (int64_t)*sp;
+            *sp = 0ull;
+        } else {
+            d = cnt[l][a];
         }
-        const uint32_t om = off_M(nl);
-        rec[om + 0] += M;
-        rec[om + 1] += M < 0 ? -M : M;
-        rec[om + 2] += M * M;
-        rec[om + 3] += 1;
+        S[l][a] = (a == 2u ? nlv : 2 * nlv) - 2 * d;
     }
+    __syncthreads();
+    const uint32_t q = lat.state[1] + q_off;
+    const uint32_t bin = q / (uint32_t)lat.sweeps_per_bin;
+    const uint32_t rs = record_size(nl);
+    int64_t *rec = lat.records + (size_t)bin * rs;
+    for (uint32_t j = threadIdx.x; j < rs; j += blockDim.x) rec[j] += record_term(j, nl, S, Msh);
 }
 
 // Advances the chain's sweep counter and sample counter after a chunk.
@@ -534,16 +583,22 @@ hipError_t launch_int8_measure0(const HbmLattice *d, int n_lats, int L, uint32_t
                                 hipStream_t s)
 {
     const uint32_t Lb = (uint32_t)L / 2u;
-    const uint32_t n_items = L >= 64 ? Lb * ((uint32_t)L / 64u) : cdiv(Lb * Lb, 32u);
+    const uint32_t n_items = L >= 32 ? Lb * ((uint32_t)L / 32u) : cdiv(Lb * Lb, 32u);
     hipLaunchKernelGGL(k_int8_measure0, dim3(cdiv(n_items, 256u), (uint32_t)n_lats), dim3(256), 0,
                        s, d, t_off);
     return hipGetLastError();
 }
 
-hipError_t launch_levels_accumulate(const HbmLattice *d, int n_lats, uint32_t t_off,
+hipError_t launch_levels_accumulate(const HbmLattice *d, int n_lats, int L, int n_levels, uint32_t t_off,
                                     uint32_t q_off, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_levels_accumulate, dim3((uint32_t)n_lats), dim3(1024), 0, s, d, t_off,
+    const uint32_t g = n_grid_levels((uint32_t)L, (uint32_t)n_levels);
+    for (uint32_t l = 1; l <= g; ++l) {
+        const uint32_t Ll = (uint32_t)L >> l;
+        hipLaunchKernelGGL(k_rm_level, dim3(cdiv(Ll * (Ll / 32u), 256u), (uint32_t)n_lats), dim3(256), 0, s, d,
+                           l, t_off);
+    }
+    hipLaunchKernelGGL(k_levels_accumulate, dim3((uint32_t)n_lats), dim3(kLevelsThreads), 0, s, d, t_off,
                        q_off);
     return hipGetLastError();
 }

@@ -10,10 +10,32 @@
 namespace mcrg {
 
 constexpr uint32_t kMaxLevels = 15;
-// LDS for levels >= 2 of the HBM schedule: level 2 of an L = 2048 lattice is
-// 512 x 512 bits = 8192 words.
-constexpr uint32_t kLdsLevelWords = 8192;
-constexpr int kMaxHbmMeasureL = 2048;
+// Blocked levels of side >= kGridLevelMinL are measured and blocked by a grid
+// of workgroups (k_rm_level, one launch per level); the smaller ones by one
+// workgroup per lattice (k_levels_accumulate), whose first level (side <= 256,
+// 2048 words) is read from global memory and the rest kept in LDS.
+constexpr uint32_t kGridLevelMinL = 512;
+constexpr uint32_t kWgLevelMaxL = kGridLevelMinL / 2;
+// Counters per lattice: level-0 dNN, dNNN, dP, ndown, then dNN, dNNN, dP of
+// every grid-measured level l >= 1 at 4 + 3 (l - 1).
+constexpr uint32_t kSumWords = 4u + 3u * kMaxLevels;
+
+// Words of a bit-packed level of side Ll (row-major bits).
+__host__ __device__ constexpr uint32_t level_words(uint32_t Ll) { return (Ll * Ll + 31u) / 32u; }
+// Word offset of level l >= 1 of a side-L lattice in its level buffer.
+__host__ __device__ inline uint32_t level_offset(uint32_t L, uint32_t l)
+{
+    uint32_t o = 0;
+    for (uint32_t j = 1; j < l; ++j) o += level_words(L >> j);
+    return o;
+}
+// Levels 1..n_grid_levels(L, nl) are grid-measured.
+__host__ __device__ inline uint32_t n_grid_levels(uint32_t L, uint32_t nl)
+{
+    uint32_t g = 0;
+    while (g < nl && (L >> (g + 1u)) >= kGridLevelMinL) ++g;
+    return g;
+}
 
 // Per-bin record of one chain measured on levels 0..nl (the reference keeps the
 // same sums as FP64, src/mcrg.cpp:59-66, with 3x3 blocks flattened column-major
@@ -39,8 +61,8 @@ __host__ __device__ constexpr uint32_t record_size(uint32_t nl) { return off_M(n
 
 struct HbmLattice {
     int8_t *spins;              // L*L int8, row-major (x fastest)
-    uint32_t *lvl1;             // level-1 bit-packed lattice, (L/2)^2 bits
-    unsigned long long *sums0;  // level-0 disagreement counts dNN, dNNN, dP, ndown
+    uint32_t *lvl;              // bit-packed levels 1.. at level_offset(L, l)
+    unsigned long long *sums;   // kSumWords disagreement counters
     int64_t *records;           // max_bins * record_size(n_levels)
     uint32_t *state;            // [0] sweep counter t, [1] measured-sample counter
     uint32_t k0, k1;            // Philox key = seed
@@ -62,7 +84,9 @@ hipError_t launch_int8_half_sweep(const HbmLattice *d, int n_lats, int L, uint32
                                   uint32_t colour, hipStream_t s);
 hipError_t launch_int8_measure0(const HbmLattice *d, int n_lats, int L, uint32_t t_off,
                                 hipStream_t s);
-hipError_t launch_levels_accumulate(const HbmLattice *d, int n_lats, uint32_t t_off,
+// Levels 1..nl (grid launches for the large ones, then one workgroup per
+// lattice) and the bin record update.
+hipError_t launch_levels_accumulate(const HbmLattice *d, int n_lats, int L, int n_levels, uint32_t t_off,
                                     uint32_t q_off, hipStream_t s);
 hipError_t launch_advance(const HbmLattice *d, int n_lats, uint32_t sweeps, uint32_t samples,
                           hipStream_t s);

@@ -412,8 +412,6 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
     int n_match = 1;
     while (n_match < 4 && (L_small >> n_match) >= 16) ++n_match;
     const int nl_small = n_match - 1, nl_large = k + nl_small, deep = n_match - 1;
-    if (nl_large > 1 && L_large > 2048)
-        throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "L_large above 2048 allows one blocking only");
     const int rs_l = mcrg_record_size(nl_large), rs_s = mcrg_record_size(nl_small);
     const int per_l = n_large / n_k, per_s = n_small / n_k;
     // jackknife groups of independent chains: replica r of every coupling and

@@ -108,7 +108,7 @@ def test_incremental_runs(amd, ctx):
 
 
 @pytest.mark.parametrize("bad,msg", [(dict(L=12), "power of two"), (dict(L=32768), "[2, 16384]"),
-                                     (dict(L=4096), "L <= 2048"), (dict(L=4), "n_levels"),
+                                     (dict(L=4), "n_levels"),
                                      (dict(n_levels=6), "n_levels"), (dict(coupling=-0.1), "coupling"),
                                      (dict(sweeps_per_bin=0), "sweeps_per_bin")])
 def test_invalid_jobs(amd, ctx, bad, msg):
