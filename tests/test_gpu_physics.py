@@ -1,4 +1,5 @@
-"""Physics parity with the reference's published MCRG results.
+"""Physics parity with the reference's published MCRG results, and with the
+exact finite-lattice energy (Kaufman's partition function, tests/exact_ising.py).
 
 The reference publishes nu = ln 2 / ln lambda per blocking level for N = 16 at
 the exact critical coupling, NN + NNN operators, b = 2 majority blocking
@@ -15,6 +16,8 @@ import math
 
 import numpy as np
 import pytest
+
+from exact_ising import nn_mean
 
 pytestmark = pytest.mark.gpu
 
@@ -35,20 +38,63 @@ def ctx(amd):
     c.close()
 
 
-def _nu(amd, ctx, sampler, n_rep, n_therm, n_meas, groups):
-    jobs = [dict(L=16, coupling=KC, seed=90000 + i, n_therm=n_therm, n_meas=n_meas, n_levels=3,
+def _replicas(amd, ctx, sampler, n_rep, n_therm, n_meas, L=16, J=KC, nl=3):
+    """One record row (one bin) per independent replica."""
+    jobs = [dict(L=L, coupling=J, seed=90000 + i, n_therm=n_therm, n_meas=n_meas, n_levels=nl,
                  sweeps_per_bin=n_meas, sampler=sampler) for i in range(n_rep)]
     with amd.Batch(ctx, jobs) as b:
         b.run_all()
-        recs = np.stack([b.records(j)[0] for j in range(n_rep)])
+        return np.stack([b.records(j)[0] for j in range(n_rep)])
+
+
+_RUNS = {}
+
+
+def _sw16(amd, ctx):
+    if "sw16" not in _RUNS:
+        print("physics: swendsen-wang L=16, 2048 replicas", flush=True)
+        _RUNS["sw16"] = _replicas(amd, ctx, amd.SAMPLER_SWENDSEN_WANG, 2048, 500, 10000)
+    return _RUNS["sw16"]
+
+
+def _metro16(amd, ctx):
+    if "metro16" not in _RUNS:
+        print("physics: metropolis L=16 (LDS-resident), 1024 replicas", flush=True)
+        _RUNS["metro16"] = _replicas(amd, ctx, amd.SAMPLER_METROPOLIS, 1024, 10000, 50000)
+    return _RUNS["metro16"]
+
+
+def _nu(amd, recs, groups):
     # independent replicas summed into jackknife groups
-    grp = recs.reshape(groups, n_rep // groups, -1).sum(axis=1)
+    grp = recs.reshape(groups, recs.shape[0] // groups, -1).sum(axis=1)
     return [amd.rg_exponent(grp, 3, lvl, 2) for lvl in range(3)]
 
 
+def _check_nn(amd, recs, L, J, nl, n_sigma=5.0):
+    """Replica means of NN per site against Kaufman's exact finite-lattice value."""
+    off_s, off_n = 0, 12 * (nl + 1) + 10 * nl + 3  # S[0][NN] and the sample count
+    assert amd.record_size(nl) == off_n + 1
+    per = recs[:, off_s] / recs[:, off_n] / (L * L)
+    mean, err = per.mean(), per.std(ddof=1) / math.sqrt(len(per))
+    exact = nn_mean(L, J) / (L * L)
+    print(f"NN/site L={L} J={J:.5f}: {mean:.6f} +- {err:.6f} (exact {exact:.6f})", flush=True)
+    assert abs(mean - exact) < n_sigma * err, (mean, err, exact)
+
+
+def test_nn_swendsen_wang_matches_kaufman(amd, ctx):
+    _check_nn(amd, _sw16(amd, ctx), 16, KC, 3)
+    print("physics: swendsen-wang L=32 J=0.42", flush=True)
+    _check_nn(amd, _replicas(amd, ctx, amd.SAMPLER_SWENDSEN_WANG, 512, 500, 5000, L=32, J=0.42, nl=1), 32, 0.42, 1)
+
+
+def test_nn_metropolis_batched_matches_kaufman(amd, ctx):
+    _check_nn(amd, _metro16(amd, ctx), 16, KC, 3)
+    print("physics: metropolis L=32 J=0.46", flush=True)
+    _check_nn(amd, _replicas(amd, ctx, amd.SAMPLER_METROPOLIS, 512, 5000, 20000, L=32, J=0.46, nl=1), 32, 0.46, 1)
+
+
 def test_nu_swendsen_wang_matches_published_n16(amd, ctx):
-    print("physics: swendsen-wang L=16", flush=True)
-    res = _nu(amd, ctx, amd.SAMPLER_SWENDSEN_WANG, 2048, 500, 10000, 64)
+    res = _nu(amd, _sw16(amd, ctx), 64)
     for lvl, (r, pub) in enumerate(zip(res, NU_PUBLISHED_N16)):
         print(f"level {lvl}: nu = {r['nu']:.5f} +- {r['nu_err']:.5f} (published {pub:.5f})", flush=True)
         assert 0 < r["nu_err"] < 0.001
@@ -56,8 +102,7 @@ def test_nu_swendsen_wang_matches_published_n16(amd, ctx):
 
 
 def test_nu_metropolis_batched_matches_published_n16(amd, ctx):
-    print("physics: metropolis L=16 (LDS-resident)", flush=True)
-    res = _nu(amd, ctx, amd.SAMPLER_METROPOLIS, 1024, 10000, 50000, 64)
+    res = _nu(amd, _metro16(amd, ctx), 64)
     for lvl, (r, pub) in enumerate(zip(res, NU_PUBLISHED_N16)):
         print(f"level {lvl}: nu = {r['nu']:.5f} +- {r['nu_err']:.5f} (published {pub:.5f})", flush=True)
         assert 0 < r["nu_err"] < 0.004
