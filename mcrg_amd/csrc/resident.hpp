@@ -53,7 +53,7 @@ struct SlotLayout {
     uint32_t team_waves;  // waves per lattice team (count partials per level)
     uint32_t n_sv;        // operator values per lattice: 3 (nl+1) + M, |M|, 1
     uint32_t coin_calls;  // Philox calls for all tie-coin words (4 words per call)
-    uint32_t meta, spins, lvl, coin, cnt, pairs, rec, total;
+    uint32_t meta, spins, lvl, coin, cnt, sv, pairs, rec, total;
 };
 
 __host__ __device__ inline SlotLayout slot_layout(uint32_t L, uint32_t G, uint32_t nl, uint32_t T)
@@ -73,7 +73,8 @@ __host__ __device__ inline SlotLayout slot_layout(uint32_t L, uint32_t G, uint32
     s.coin = s.lvl + G * s.lvl_words;
     s.cnt = s.coin + G * s.lvl_words;
     s.n_sv = 3u * (nl + 1u) + 3u;
-    s.pairs = s.cnt + G * s.team_waves * (nl + 1u) * 4u;  // one (x, y) operand pair per record entry
+    s.sv = s.cnt + G * s.team_waves * (nl + 1u) * 4u;  // int32 operator values (teams within one wave)
+    s.pairs = s.sv + G * s.n_sv;                         // one (x, y) operand pair per record entry
     s.rec = (s.pairs + record_size(nl) + 1u) & ~1u;  // int64 aligned
     s.total = s.rec + G * record_size(nl) * 2u;
     return s;

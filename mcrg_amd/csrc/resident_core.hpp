@@ -328,6 +328,7 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
     uint32_t *lvl = lds + lay.lvl;
     uint32_t *coins = lds + lay.coin;
     uint32_t *cnt = lds + lay.cnt;
+    int32_t *sv = reinterpret_cast<int32_t *>(lds + lay.sv);
     int64_t *rec = reinterpret_cast<int64_t *>(lds + lay.rec);
     uint32_t *pairs = lds + lay.pairs;
 
@@ -512,10 +513,21 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
         }
 #endif
 
-        // ---- the sample's sums and products into the partial records; each
-        //      entry forms its two operator values from the counts.  No barrier
-        //      follows: the next writes of the counts (level 0 of the next
-        //      sample) come after the two sweep barriers.
+        // ---- the sample's sums and products into the partial records.  A
+        //      team within one wave first forms each operator value once (its
+        //      barrier is a wave barrier); a team spanning waves lets every
+        //      entry form its two values from the counts, which saves a
+        //      workgroup barrier.  No barrier follows: the next writes of the
+        //      counts and values (next sample) come after the two sweep barriers.
+        if (!tm.block) {
+            for (uint32_t g = tm.g0; g < G; g += tm.gs) {
+                if (!in_meas(meta + g * kMetaWords, it)) continue;
+                const uint32_t *cntg = cnt + g * cstride;
+                for (uint32_t u = tm.u0; u < T; u += tm.us)
+                    for (uint32_t i = u; i < lay.n_sv; i += T) sv[g * lay.n_sv + i] = (int32_t)sv_value(cntg, nw, nl, L, i);
+            }
+            team_sync(tm);
+        }
         for (uint32_t g = tm.g0; g < G; g += tm.gs) {
             const uint32_t *m = meta + g * kMetaWords;
             if (!in_meas(m, it)) continue;
@@ -531,7 +543,10 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
                         r = 0;
                     }
                     const uint32_t xy = pairs[k];
-                    r += sv_value(cntg, nw, nl, L, xy & 0xFFFFu) * sv_value(cntg, nw, nl, L, xy >> 16);
+                    if (tm.block)
+                        r += sv_value(cntg, nw, nl, L, xy & 0xFFFFu) * sv_value(cntg, nw, nl, L, xy >> 16);
+                    else
+                        r += (int64_t)sv[g * lay.n_sv + (xy & 0xFFFFu)] * (int64_t)sv[g * lay.n_sv + (xy >> 16)];
                 }
             }
         }
