@@ -91,20 +91,22 @@ __device__ __forceinline__ bool in_meas(const uint32_t *m, uint32_t it)
     return it >= m[kMetaTherm] && it < m[kMetaTherm] + m[kMetaMeas];
 }
 
-// Operator counts of lattice g at level l: sum over the team's wave partials.
+// Operator counts of lattice g at level l: sum over the team's wave partials;
+// levels >= lf were measured by the team's first wave alone.
 __device__ __forceinline__ uint32_t count_of(const uint32_t *cntg, uint32_t nw, uint32_t nl, uint32_t l,
-                                             uint32_t a)
+                                             uint32_t a, uint32_t lf)
 {
+    if (l >= lf) nw = 1u;
     uint32_t s = 0;
     for (uint32_t w = 0; w < nw; ++w) s += cntg[(w * (nl + 1u) + l) * 4u + a];
     return s;
 }
 
 __device__ __forceinline__ int64_t level_S(const uint32_t *cntg, uint32_t nw, uint32_t nl, uint32_t L,
-                                           uint32_t l, uint32_t a)
+                                           uint32_t l, uint32_t a, uint32_t lf)
 {
     const int64_t n = (int64_t)(L >> l) * (int64_t)(L >> l);
-    const int64_t d = (int64_t)count_of(cntg, nw, nl, l, a);
+    const int64_t d = (int64_t)count_of(cntg, nw, nl, l, a, lf);
     return a < 2u ? 2 * n - 2 * d : n - 2 * d;
 }
 
@@ -138,13 +140,16 @@ __device__ __forceinline__ void record_pair(uint32_t nl, uint32_t k, uint32_t &x
 
 // Operator value i of sv (see record_pair) from the counts.  |value| <= 2 L^2
 // <= 2^19 for L <= 512, so the slot keeps them as int32.
-__device__ __forceinline__ int64_t sv_value(const uint32_t *cntg, uint32_t nw, uint32_t nl, uint32_t L, uint32_t i)
+// Levels >= lf come from the first wave's partial only (the register pyramid);
+// lf = nl + 1 sums every level over all partials.
+__device__ __forceinline__ int64_t sv_value(const uint32_t *cntg, uint32_t nw, uint32_t nl, uint32_t L, uint32_t i,
+                                            uint32_t lf)
 {
     const uint32_t n_s = 3u * (nl + 1u);
-    if (i < n_s) return level_S(cntg, nw, nl, L, i / 3u, i % 3u);
+    if (i < n_s) return level_S(cntg, nw, nl, L, i / 3u, i % 3u, lf);
     if (i == n_s + 2u) return 1;
     const int64_t n0 = (int64_t)L * (int64_t)L;
-    const int64_t M = n0 - 2 * (int64_t)count_of(cntg, nw, nl, 0u, 3u);
+    const int64_t M = n0 - 2 * (int64_t)count_of(cntg, nw, nl, 0u, 3u, lf);
     return i == n_s ? M : (M < 0 ? -M : M);
 }
 
@@ -499,32 +504,41 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
                 const bool meas = in_meas(meta + g * kMetaWords, it);
                 for (uint32_t u = tm.u0; u < T; u += tm.us) {
                     uint32_t *cntg = cnt + g * cstride + (u >> 6) * (nl + 1u) * 4u;
-                    if (u < 64u) {
-                        if (meas)
-                            pyramid(lvl + g * lay.lvl_words, coins + g * lay.lvl_words, cntg, L, pl, nl, off,
-                                    coin_off, u, T < 64u ? T : 64u);
-                    } else if ((u & 63u) == 0u) {
-                        for (uint32_t l = pl; l <= nl; ++l)
-                            for (uint32_t a = 0; a < 4u; ++a) cntg[l * 4u + a] = 0u;
+                    if (u < 64u && meas) {
+                        pyramid(lvl + g * lay.lvl_words, coins + g * lay.lvl_words, cntg, L, pl, nl, off,
+                                coin_off, u, T < 64u ? T : 64u);
+                        if (tm.block) {
+                            // a team spanning waves: its first wave, which holds
+                            // the pyramid's counts, forms the operator values
+                            // (the other partials were published by earlier barriers)
+                            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                            __builtin_amdgcn_wave_barrier();
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                            for (uint32_t i = u; i < lay.n_sv; i += 64u)
+                                sv[g * lay.n_sv + i] = (int32_t)sv_value(cntg, nw, nl, L, i, pl);
+                        }
                     }
                 }
             }
             team_sync(tm);
         }
 #endif
+        // operator values in `sv`: always for teams within a wave, and for
+        // wider teams when the pyramid ran (else each entry forms its own)
+        const bool sv_ready = !tm.block || pl != 0u;
 
         // ---- the sample's sums and products into the partial records.  A
         //      team within one wave first forms each operator value once (its
-        //      barrier is a wave barrier); a team spanning waves lets every
-        //      entry form its two values from the counts, which saves a
-        //      workgroup barrier.  No barrier follows: the next writes of the
-        //      counts and values (next sample) come after the two sweep barriers.
+        //      barrier is a wave barrier).  No barrier follows: the next writes
+        //      of the counts and values (next sample) come after the two sweep
+        //      barriers.
         if (!tm.block) {
             for (uint32_t g = tm.g0; g < G; g += tm.gs) {
                 if (!in_meas(meta + g * kMetaWords, it)) continue;
                 const uint32_t *cntg = cnt + g * cstride;
                 for (uint32_t u = tm.u0; u < T; u += tm.us)
-                    for (uint32_t i = u; i < lay.n_sv; i += T) sv[g * lay.n_sv + i] = (int32_t)sv_value(cntg, nw, nl, L, i);
+                    for (uint32_t i = u; i < lay.n_sv; i += T)
+                        sv[g * lay.n_sv + i] = (int32_t)sv_value(cntg, nw, nl, L, i, nl + 1u);
             }
             team_sync(tm);
         }
@@ -543,10 +557,10 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
                         r = 0;
                     }
                     const uint32_t xy = pairs[k];
-                    if (tm.block)
-                        r += sv_value(cntg, nw, nl, L, xy & 0xFFFFu) * sv_value(cntg, nw, nl, L, xy >> 16);
-                    else
+                    if (sv_ready)
                         r += (int64_t)sv[g * lay.n_sv + (xy & 0xFFFFu)] * (int64_t)sv[g * lay.n_sv + (xy >> 16)];
+                    else
+                        r += sv_value(cntg, nw, nl, L, xy & 0xFFFFu, nl + 1u) * sv_value(cntg, nw, nl, L, xy >> 16, nl + 1u);
                 }
             }
         }
