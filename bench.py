@@ -50,17 +50,24 @@ sys.path.insert(0, ROOT)
 
 METRIC = "spin-updates/ns (batched vs plain) at matched y_t error; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of k_resident<threads> for (replicas, sweeps per launch,
-# threads), from two rocprofv3 --pmc passes over `bench.py --only batched`:
+# HBM bytes per launch of k_resident<threads>, keyed by (replicas, sweeps per
+# launch, threads, min_team, max_slot_lds, threads_large), from two rocprofv3
+# --pmc passes over `bench.py --only batched` with the same configuration flags:
 # FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 KiB per dispatch, the scale factors read
 # off the same counters on a 1 GiB read + 1 GiB write copy kernel
-# (--copy-calibration).  Other shapes report null.
-MEASURED_TRAFFIC = {(4096, 100, 256): (3151.5 * 2.0 + 5051.8) * 1024.0}
+# (--copy-calibration).  Configurations without a counter run report null.
+MEASURED_TRAFFIC = {
+    (4096, 100, 256, 16, 0, 0): (3150.9 * 2.0 + 5051.8) * 1024.0,
+    (4096, 100, 256, 16, 65536, 0): (3153.6 * 2.0 + 5051.8) * 1024.0,
+}
 # The same kernel is bound by VALU issue, not HBM: rocprofv3 --pmc SQ_INSTS_VALU
 # and GRBM_GUI_ACTIVE over the same launches give the wave-level VALU
 # instructions per launch and the busy cycles per XCD (8 XCDs, 1024 SIMDs that
 # issue one wave64 VALU instruction per 4 cycles).
-MEASURED_VALU = {(4096, 100, 256): {"valu_wave_instr_per_launch": 6.354e8, "gui_active_cycles_per_xcd": 35491179.6 / 8}}
+MEASURED_VALU = {
+    (4096, 100, 256, 16, 0, 0): {"valu_wave_instr_per_launch": 6.212e8, "gui_active_cycles_per_xcd": 33963140.6 / 8},
+    (4096, 100, 256, 16, 65536, 0): {"valu_wave_instr_per_launch": 6.212e8, "gui_active_cycles_per_xcd": 34290596.0 / 8},
+}
 
 
 def log(*a):
@@ -73,12 +80,13 @@ def bytes_per_update(measured):
     return 3.0 / 8.0 + (5.0 / 24.0 if measured else 0.0)
 
 
-def run_batched(mcrg_amd, ctx, jobs, steps, warmup, sps, threads, autotune=False):
-    """The autotuner (when on) tries its candidates during the warmup steps."""
+def run_batched(mcrg_amd, ctx, jobs, steps, warmup, sps, threads, autotune=False, cfg=None):
+    """The autotuner (when on) tries its candidates during the warmup steps;
+    otherwise `cfg` (min_team, max_slot_lds, threads_large) fixes the configuration."""
     n_sweeps = (warmup + steps) * sps
     bjobs = [dict(j, n_therm=0, n_meas=n_sweeps, sweeps_per_bin=max(1, n_sweeps)) for j in jobs]
     sites = sum(j["L"] * j["L"] for j in bjobs)
-    with mcrg_amd.Batch(ctx, bjobs, threads=threads, sweeps_per_launch=sps, autotune=autotune) as b:
+    with mcrg_amd.Batch(ctx, bjobs, threads=threads, sweeps_per_launch=sps, autotune=autotune, **(cfg or {})) as b:
         b.run(warmup * sps)
         b.sync()
         opts = b.options()
@@ -283,6 +291,9 @@ def main():
                     help="seconds after the last arrival to wait for completions")
     ap.add_argument("--serving-quantum", type=int, default=0)
     ap.add_argument("--bisect-rounds", type=int, default=8, help="rounds of the bisection leg (0 = skip)")
+    ap.add_argument("--min-team", type=int, default=16, help="--only batched: lanes per lattice at least")
+    ap.add_argument("--max-slot-lds", type=int, default=0, help="--only batched: LDS cap per slot (0 = default)")
+    ap.add_argument("--threads-large", type=int, default=0, help="--only batched: workgroup size for L = 512")
     ap.add_argument("--only", choices=["batched", "plain_l2048"], default=None,
                     help="run one leg alone and print its line to stderr (for profiler runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -310,9 +321,10 @@ def main():
     jobs = workloads.batch_manifest(args.replicas)
     ctx = mcrg_amd.Context(0)
     if args.only == "batched":
+        cfg = dict(min_team=args.min_team, max_slot_lds=args.max_slot_lds, threads_large=args.threads_large)
         value, dt, _, _ = run_batched(mcrg_amd, ctx, jobs, args.steps, args.warmup, args.sweeps_per_step,
-                                      args.threads)
-        log(f"bench: batched {value:.1f} updates/ns ({dt:.3f} s timed)")
+                                      args.threads, cfg=cfg)
+        log(f"bench: batched {value:.1f} updates/ns ({dt:.3f} s timed), {cfg}")
         ctx.close()
         return 0
     if args.only == "plain_l2048":
@@ -329,8 +341,8 @@ def main():
         f"threads_large={tuned['threads_large']}")
 
     avg_ms, n_launch = run_batched_timed(mcrg_amd, ctx, jobs, args.sweeps_per_step, tuned, 11)
-    default_cfg = (tuned["threads"], tuned["min_team"], tuned["max_slot_lds"], tuned["threads_large"]) == (256, 16, 0, 0)
-    cfg_key = (len(jobs), args.sweeps_per_step, 256) if default_cfg else None
+    cfg_key = (len(jobs), args.sweeps_per_step, tuned["threads"], tuned["min_team"], tuned["max_slot_lds"],
+               tuned["threads_large"])
     bpl = bytes_per_update(True) * sites * args.sweeps_per_step
     achieved = bpl / (avg_ms * 1e-3) / 1e9
     log(f"bench: resident kernel {avg_ms:.3f} ms/launch over {n_launch} launches -> {achieved:.1f} GB/s")
