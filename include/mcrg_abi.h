@@ -191,7 +191,7 @@ typedef struct {
     int32_t max_team;          /* lanes per lattice at most (power of two,
                                   >= min_team); 0 = threads                     */
     int32_t threads_large;     /* workgroup size for lattices of L = 512
-                                  (64/128/256/512); 0 = threads                 */
+                                  (64/128/256/512/1024); 0 = threads            */
     int32_t autotune;          /* nonzero: the first launches each try one
                                   candidate configuration (timed with events)
                                   and the fastest is kept; results are the same
@@ -243,7 +243,7 @@ typedef struct {
     int32_t max_jobs;          /* jobs resident at once; 0 = 65536                    */
     int32_t threads;           /* workgroup size; 0 = 256                             */
     int32_t sweeps_per_launch; /* most sweeps of one job per launch; 0 = 65536        */
-    int32_t threads_large;     /* workgroup size for L = 512 jobs; 0 = 512            */
+    int32_t threads_large;     /* workgroup size for L = 512 jobs; 0 = 1024           */
 } mcrg_server_opts;
 
 enum { MCRG_JOB_QUEUED = 0, MCRG_JOB_RUNNING = 1, MCRG_JOB_DONE = 2, MCRG_JOB_CANCELLED = 3 };

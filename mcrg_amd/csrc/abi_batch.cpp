@@ -78,8 +78,8 @@ mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n
         eo.threads_large = opts->threads_large;
     }
     MCRG_REQUIRE(eo.threads_large == 0 || eo.threads_large == 64 || eo.threads_large == 128 ||
-                     eo.threads_large == 256 || eo.threads_large == 512,
-                 "threads_large must be 0, 64, 128, 256 or 512");
+                     eo.threads_large == 256 || eo.threads_large == 512 || eo.threads_large == 1024,
+                 "threads_large must be 0, 64, 128, 256, 512 or 1024");
     MCRG_REQUIRE(eo.max_team == 0 || (eo.max_team >= eo.min_team && eo.max_team <= eo.threads &&
                                       (eo.max_team & (eo.max_team - 1)) == 0),
                  "max_team must be 0 or a power of two in [min_team, threads]");

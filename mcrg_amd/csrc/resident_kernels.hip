@@ -98,6 +98,7 @@ hipError_t launch_resident(int threads, const Slot *slots, uint32_t n_slots, con
     case 128: return launch_nt<128>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
     case 256: return launch_nt<256>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
     case 512: return launch_nt<512>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
+    case 1024: return launch_nt<1024>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -111,6 +112,7 @@ int resident_max_active_groups(int threads, uint32_t lds_bytes)
     case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resident<128>, 128, lds_bytes); break;
     case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resident<256>, 256, lds_bytes); break;
     case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resident<512>, 512, lds_bytes); break;
+    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_resident<1024>, 1024, lds_bytes); break;
     default: break;
     }
     if (e != hipSuccess || per_cu <= 0) return 0;
