@@ -226,8 +226,10 @@ mcrg_status mcrg_server_create(mcrg_ctx ctx, const mcrg_server_opts *opts, mcrg_
     eo.sweeps_per_launch = 65536;
     eo.quantum_ns = 4000000;  // 4 ms of work per workgroup and launch
     // one L = 512 lattice sweeps 1.2x faster on 1024 lanes than on 512 (and 1.5x
-    // faster on 512 than on 256): the longest jobs, which set the tail latency
+    // faster on 512 than on 256); L = 256 and 512 jobs are the longest and set
+    // the tail latency
     eo.threads_large = 1024;
+    eo.large_min_L = 256;
     if (opts) {
         if (opts->hbm_budget_bytes) budget = opts->hbm_budget_bytes;
         if (opts->lds_budget_bytes) lds_budget = opts->lds_budget_bytes;
