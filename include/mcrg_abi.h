@@ -243,7 +243,9 @@ typedef struct {
     int32_t max_jobs;          /* jobs resident at once; 0 = 65536                    */
     int32_t threads;           /* workgroup size; 0 = 256                             */
     int32_t sweeps_per_launch; /* most sweeps of one job per launch; 0 = 65536        */
-    int32_t threads_large;     /* workgroup size for L >= 256 jobs; 0 = 1024          */
+    int32_t threads_large;     /* workgroup size for L >= 256 jobs; 0 = adaptive:
+                                  1024 while no job waits for admission, 512
+                                  while some do (saturated device)            */
 } mcrg_server_opts;
 
 enum { MCRG_JOB_QUEUED = 0, MCRG_JOB_RUNNING = 1, MCRG_JOB_DONE = 2, MCRG_JOB_CANCELLED = 3 };

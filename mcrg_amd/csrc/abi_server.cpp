@@ -49,6 +49,7 @@ struct mcrg_server_s {
     std::deque<uint64_t> completed;
     uint64_t next_id = 0;
     int64_t lds_budget = 0, lds_in_use = 0;
+    bool adaptive_large = false;  // large-lattice workgroup size follows the queue
     std::thread th;
     std::atomic<bool> stop{false};
     bool started = false;
@@ -106,6 +107,12 @@ static mcrg_status server_step(mcrg_server s, int32_t *active)
             s->running.push_back(id);
             s->queue.pop_front();
         }
+        // Latency mode while every job is admitted: the long L >= 256 jobs get
+        // 1024-lane workgroups (each lattice sweeps 1.2x faster).  Once jobs
+        // wait for room the device is saturated and 512-lane workgroups give
+        // more aggregate updates (2237 against 1864 updates/ns with 512
+        // resident L = 512 lattices).
+        if (s->adaptive_large) s->engine.set_threads_large(s->queue.empty() ? 1024 : 512);
         s->st.queued = (int64_t)s->queue.size();
         s->st.running = (int64_t)s->running.size();
         s->st.hbm_in_use = (int64_t)s->engine.arena_in_use();
@@ -225,11 +232,13 @@ mcrg_status mcrg_server_create(mcrg_ctx ctx, const mcrg_server_opts *opts, mcrg_
     EngineOptions eo;
     eo.sweeps_per_launch = 65536;
     eo.quantum_ns = 4000000;  // 4 ms of work per workgroup and launch
-    // one L = 512 lattice sweeps 1.2x faster on 1024 lanes than on 512 (and 1.5x
-    // faster on 512 than on 256); L = 256 and 512 jobs are the longest and set
-    // the tail latency
+    // L = 256 and 512 jobs are the longest and set the tail latency: one
+    // L = 512 lattice sweeps 1.2x faster on 1024 lanes than on 512 (and 1.5x
+    // faster on 512 than on 256).  Unless the caller fixes it, the size
+    // follows the queue (server_step).
     eo.threads_large = 1024;
     eo.large_min_L = 256;
+    bool adaptive = true;
     if (opts) {
         if (opts->hbm_budget_bytes) budget = opts->hbm_budget_bytes;
         if (opts->lds_budget_bytes) lds_budget = opts->lds_budget_bytes;
@@ -239,7 +248,10 @@ mcrg_status mcrg_server_create(mcrg_ctx ctx, const mcrg_server_opts *opts, mcrg_
         if (opts->sweeps_per_launch) eo.sweeps_per_launch = opts->sweeps_per_launch;
         MCRG_REQUIRE(opts->launch_quantum >= 0, "launch_quantum must be >= 0");
         MCRG_REQUIRE(opts->lds_budget_bytes >= 0, "lds_budget_bytes must be >= 0");
-        if (opts->threads_large) eo.threads_large = opts->threads_large;
+        if (opts->threads_large) {
+            eo.threads_large = opts->threads_large;
+            adaptive = false;
+        }
     }
     MCRG_REQUIRE(budget >= 4096, "hbm_budget_bytes must be >= 4096");
     MCRG_REQUIRE(max_jobs >= 1, "max_jobs must be >= 1");
@@ -264,6 +276,7 @@ mcrg_status mcrg_server_create(mcrg_ctx ctx, const mcrg_server_opts *opts, mcrg_
     s->ctx = ctx;
     s->eo = eo;
     s->lds_budget = lds_budget;
+    s->adaptive_large = adaptive;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = s->engine.init(ctx->device, (size_t)budget, max_jobs, eo, s->stream);
     if (e != hipSuccess) {

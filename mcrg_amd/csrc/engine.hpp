@@ -112,6 +112,14 @@ class ResidentEngine {
     void enable_autotune();
     bool tuning() const { return tune_next_ < tune_.size(); }
     const EngineOptions &options() const { return opt_; }
+    // Workgroup size of the large-lattice class from the next launch on.
+    void set_threads_large(int threads)
+    {
+        if (threads != opt_.threads_large) {
+            opt_.threads_large = threads;
+            slots_dirty_ = true;
+        }
+    }
     int group_size(int L, int nl) const;
     // Workgroup size of the slots of side L.
     int threads_for(int L) const
