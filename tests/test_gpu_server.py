@@ -146,6 +146,31 @@ def test_lds_budget_limits_residency(amd, ctx, orc):
         assert e.value.status == amd.ERR_OUT_OF_BUDGET
 
 
+def test_large_lattices_follow_the_queue(amd, ctx, orc):
+    """L = 256 and 512 jobs under an LDS budget of about one and a half L = 512
+    lattices: the large class runs on 512-lane workgroups while jobs wait and
+    on 1024-lane ones once the queue is empty, switching while jobs are
+    resident; every job equals the oracle."""
+    jobs = [dict(L=L, coupling=KC, seed=400 + i, n_therm=2, n_meas=9, n_levels=nl, sweeps_per_bin=3)
+            for i, (L, nl) in enumerate([(512, 3), (256, 4), (512, 7), (256, 2), (512, 5), (256, 6)])]
+    with amd.Server(ctx, sweeps_per_launch=4) as probe:
+        probe.submit(jobs[0])
+        probe.step()
+        share = probe.stats()["lds_in_use"]
+    assert share > 0
+    with amd.Server(ctx, lds_budget_bytes=share + share // 2, sweeps_per_launch=4) as srv:
+        ids = [srv.submit(j) for j in jobs]
+        queued_seen = False
+        for _ in range(400):
+            active = srv.step()
+            queued_seen = queued_seen or srv.stats()["queued"] > 0
+            if active == 0 and srv.stats()["queued"] == 0:
+                break
+        assert queued_seen
+        for jid, job in zip(ids, jobs):
+            assert np.array_equal(srv.records(jid), _expect(orc, job)), job
+
+
 def test_errors_and_cancel(amd, ctx):
     with amd.Server(ctx, hbm_budget_bytes=1 << 16, sweeps_per_launch=5) as srv:
         for bad in (dict(L=12, coupling=0.4, seed=1, n_meas=4, sweeps_per_bin=1),
