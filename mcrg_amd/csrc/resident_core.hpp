@@ -15,9 +15,8 @@
 //            record, which is flushed into the per-bin record in HBM when the
 //            lattice's sample index enters a new bin and at the chunk end.
 //
-// Loops are written as `for g` / `for u` with strides (gs, us): on the GPU they
-// run once per thread (g = tid / T, u = tid % T); the CPU emulation used while
-// developing runs them with stride 1 and no synchronisation.
+// Loops are written as `for g` / `for u` with strides (gs, us); each runs once
+// per thread (g = tid / T, u = tid % T).
 #pragma once
 
 #include "levels.hpp"
@@ -48,7 +47,6 @@ struct Team {
     bool block;               // team spans more than one wave
 };
 
-#ifndef MCRG_HOST_EMULATION
 __device__ __forceinline__ void team_sync(const Team &tm)
 {
     if (tm.block) {
@@ -80,7 +78,6 @@ __device__ __forceinline__ uint32_t group_sum(uint32_t v, uint32_t width)
     if (width >= 64u) v += (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
     return v;
 }
-#endif
 
 __device__ __forceinline__ bool in_run(const uint32_t *m, uint32_t it)
 {
@@ -158,25 +155,16 @@ __device__ __forceinline__ void store_counts(uint32_t *cntg, const Team &tm, uin
                                              uint32_t u, uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3)
 {
     const uint32_t width = tm.T < 64u ? tm.T : 64u;
-#ifndef MCRG_HOST_EMULATION
     d0 = group_sum(d0, width);
     d1 = group_sum(d1, width);
     d2 = group_sum(d2, width);
     d3 = group_sum(d3, width);
     if ((u & (width - 1u)) != 0u) return;
-#endif
     uint32_t *c = cntg + ((u / width) * (nl + 1u) + l) * 4u;
-#ifdef MCRG_HOST_EMULATION
-    c[0] += d0;  // the emulation visits every lane of the team in turn
-    c[1] += d1;
-    c[2] += d2;
-    c[3] += d3;
-#else
     c[0] = d0;
     c[1] = d1;
     c[2] = d2;
     c[3] = d3;
-#endif
 }
 
 // Wave-uniform value: when a lattice's team covers whole waves its key and
@@ -184,11 +172,7 @@ __device__ __forceinline__ void store_counts(uint32_t *cntg, const Team &tm, uin
 template <bool UNIFORM>
 __device__ __forceinline__ uint32_t uni(uint32_t v)
 {
-#ifdef MCRG_HOST_EMULATION
-    return v;
-#else
     return UNIFORM ? (uint32_t)__builtin_amdgcn_readfirstlane((int)v) : v;
-#endif
 }
 
 // Half-sweep of colour c for the team's lattice.
@@ -217,7 +201,6 @@ __device__ __forceinline__ void sweep_colour(const uint32_t *meta, uint32_t *spi
     }
 }
 
-#ifndef MCRG_HOST_EMULATION
 // First level whose rows fit one per lane of the team's first wave, or 0 when
 // no level from 1 to nl does (L = 8, or L = 16 with one level).
 __device__ __forceinline__ uint32_t pyramid_level(uint32_t L, uint32_t T, uint32_t nl)
@@ -317,7 +300,6 @@ __device__ __forceinline__ void pyramid(const uint32_t *lg, const uint32_t *cg, 
         Ll = Lb;
     }
 }
-#endif
 
 __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
                          PackedLattice *__restrict__ lats, uint32_t *lds, Team tm)
@@ -465,11 +447,7 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
 
         // ---- levels 1..nl: spread over the team while a level has more rows
         //      than the team's first wave has lanes, then the register pyramid
-#ifdef MCRG_HOST_EMULATION
-        const uint32_t pl = 0u;
-#else
         const uint32_t pl = pyramid_level(L, T, nl);
-#endif
         const uint32_t last_spread = pl ? pl - 1u : nl;
         uint32_t off = 0;
         for (uint32_t l = 1; l <= last_spread; ++l) {
@@ -496,7 +474,6 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
             team_sync(tm);
         }
 
-#ifndef MCRG_HOST_EMULATION
         if (pl) {
             // `off` is now the RM offset of level pl; its coins start one level later
             const uint32_t coin_off = off + level_words(L, pl);
@@ -522,7 +499,6 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
             }
             team_sync(tm);
         }
-#endif
         // operator values in `sv`: always for teams within a wave, and for
         // wider teams when the pyramid ran (else each entry forms its own)
         const bool sv_ready = !tm.block || pl != 0u;
@@ -564,9 +540,6 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
                 }
             }
         }
-#ifdef MCRG_HOST_EMULATION
-        for (uint32_t i = 0; i < G * cstride; ++i) cnt[i] = 0u;
-#endif
     }
     block_sync();
 
