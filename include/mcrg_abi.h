@@ -191,7 +191,9 @@ typedef struct {
     int32_t max_team;          /* lanes per lattice at most (power of two,
                                   >= min_team); 0 = threads                     */
     int32_t threads_large;     /* workgroup size for lattices of L = 512
-                                  (64/128/256/512/1024); 0 = threads            */
+                                  (64/128/256/512/1024); 0 = 1024 when there
+                                  are at most as many of them as compute
+                                  units, else 512                               */
     int32_t autotune;          /* nonzero: the first launches each try one
                                   candidate configuration (timed with events)
                                   and the fastest is kept; results are the same

@@ -119,7 +119,7 @@ def test_invalid_jobs(amd, ctx, bad, msg):
     assert e.value.status == amd.ERR_INVALID_ARG and msg in str(e.value)
 
 
-@pytest.mark.parametrize("threads_large", [0, 512, 1024])
+@pytest.mark.parametrize("threads_large", [0, 256, 512, 1024])
 def test_launch_classes_mixed_lds(amd, ctx, orc, threads_large):
     """Slots of very different LDS size (L = 8 groups next to single L = 512
     lattices) launch as concurrent classes, with L = 512 on 256- or 512-thread
