@@ -216,41 +216,54 @@ __global__ __launch_bounds__(256) void k_int8_half_sweep(const HbmLattice *__res
     const Accept acc = lat.acc;
     int8_t *spins = lat.spins;
     if (L >= 64) {
+        // One thread per (strip of R rows, 64-column chunk): it slides a window
+        // of row chunks down its strip, so each row is read (R + 2) / R times
+        // instead of three.  Only the other colour's bytes of the rows
+        // above and below are used, and no thread writes those in this half.
         const uint32_t wpr = L >> 6, sh = (uint32_t)__builtin_ctz(wpr);
-        if (w >= L * wpr) return;
-        const uint32_t y = w >> sh, q = w & (wpr - 1u);
-        const uint32_t p = (y + colour) & 1u;
-        int8_t *row = spins + (size_t)y * L + 64u * q;
-        const int8_t *up = spins + (size_t)((y - 1u) & (L - 1u)) * L + 64u * q;
-        const int8_t *dn = spins + (size_t)((y + 1u) & (L - 1u)) * L + 64u * q;
-        uint32_t R[16], U[16], D[16];
-        load64(row, R);
-        load64(up, U);
-        load64(dn, D);
-        const int8_t *rowbase = spins + (size_t)y * L;
-        const uint32_t s = gather_parity(R, p);
-        const uint32_t o = gather_parity(R, p ^ 1u);
-        const uint32_t nu = gather_parity(U, p);
-        const uint32_t nd = gather_parity(D, p);
-        uint32_t nl, nr;
-        if (p == 0u) {
-            const uint32_t e = sbit(rowbase[(64u * q - 1u) & (L - 1u)]);
-            nr = o;
-            nl = (o << 1) | e;
-        } else {
-            const uint32_t e = sbit(rowbase[(64u * q + 64u) & (L - 1u)]);
-            nl = o;
-            nr = (o >> 1) | (e << 31);
-        }
-        const uint32_t flip = metropolis_word(s, nu, nd, nl, nr, 0xFFFFFFFFu, acc, lat.k0, lat.k1,
-                                              w, t, colour);
-        if (flip) {
+        const uint32_t R = half_sweep_rows(L);
+        if (w >= (L / R) * wpr) return;
+        const uint32_t q = w & (wpr - 1u), y0 = (w >> sh) * R;
+        uint32_t U[16], C[16], D[16], E[16];
+        load64(spins + (size_t)((y0 - 1u) & (L - 1u)) * L + 64u * q, U);
+        load64(spins + (size_t)y0 * L + 64u * q, C);
+        load64(spins + (size_t)((y0 + 1u) & (L - 1u)) * L + 64u * q, D);
+        for (uint32_t r = 0; r < R; ++r) {
+            const uint32_t y = y0 + r;
+            // the row after next is fetched while this row is decided
+            if (r + 1u < R) load64(spins + (size_t)((y + 2u) & (L - 1u)) * L + 64u * q, E);
+            const uint32_t p = (y + colour) & 1u;
+            const int8_t *rowbase = spins + (size_t)y * L;
+            const uint32_t s = gather_parity(C, p);
+            const uint32_t o = gather_parity(C, p ^ 1u);
+            const uint32_t nu = gather_parity(U, p);
+            const uint32_t nd = gather_parity(D, p);
+            uint32_t nl, nr;
+            if (p == 0u) {
+                const uint32_t e = sbit(rowbase[(64u * q - 1u) & (L - 1u)]);
+                nr = o;
+                nl = (o << 1) | e;
+            } else {
+                const uint32_t e = sbit(rowbase[(64u * q + 64u) & (L - 1u)]);
+                nl = o;
+                nr = (o >> 1) | (e << 31);
+            }
+            const uint32_t flip = metropolis_word(s, nu, nd, nl, nr, 0xFFFFFFFFu, acc, lat.k0, lat.k1,
+                                                  y * wpr + q, t, colour);
+            if (flip) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const uint32_t f0 = (flip >> (2 * i)) & 1u, f1 = (flip >> (2 * i + 1)) & 1u;
+                    C[i] ^= ((f0 * 0xFEu) << (8u * p)) | ((f1 * 0xFEu) << (8u * p + 16u));
+                }
+                store64(spins + (size_t)y * L + 64u * q, C);
+            }
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                const uint32_t f0 = (flip >> (2 * i)) & 1u, f1 = (flip >> (2 * i + 1)) & 1u;
-                R[i] ^= ((f0 * 0xFEu) << (8u * p)) | ((f1 * 0xFEu) << (8u * p + 16u));
+                U[i] = C[i];
+                C[i] = D[i];
+                D[i] = E[i];
             }
-            store64(row, R);
         }
         return;
     }
@@ -574,7 +587,9 @@ hipError_t launch_int8_half_sweep(const HbmLattice *d, int n_lats, int L, uint32
 {
     const uint32_t n_sites = (uint32_t)L * (uint32_t)L / 2u;
     const uint32_t n_words = cdiv(n_sites, 32u);
-    hipLaunchKernelGGL(k_int8_half_sweep, dim3(cdiv(n_words, 256u), (uint32_t)n_lats), dim3(256), 0,
+    // L >= 64: one thread per strip of half_sweep_rows(L) rows and 64 columns
+    const uint32_t n_items = L >= 64 ? n_words / half_sweep_rows((uint32_t)L) : n_words;
+    hipLaunchKernelGGL(k_int8_half_sweep, dim3(cdiv(n_items, 256u), (uint32_t)n_lats), dim3(256), 0,
                        s, d, t_off, colour);
     return hipGetLastError();
 }

@@ -20,6 +20,14 @@ constexpr uint32_t kWgLevelMaxL = kGridLevelMinL / 2;
 // every grid-measured level l >= 1 at 4 + 3 (l - 1).
 constexpr uint32_t kSumWords = 4u + 3u * kMaxLevels;
 
+// Rows per thread strip of the int8 half-sweep (L >= 64).  Up to L = 8192 the
+// lattice (<= 64 MiB) stays in the 256 MB last-level cache, so each row's three
+// reads (as a row and as the neighbour of two others) cost little and one row
+// per thread keeps the most threads in flight.  The 256 MiB of L = 16384 do
+// not fit: strips of 8 rows read each row 10/8 times from HBM (131 against
+// 159 us per half-sweep; 16 rows: 131, 4: 136, 2: 143).
+__host__ __device__ inline uint32_t half_sweep_rows(uint32_t L) { return L >= 16384u ? 8u : 1u; }
+
 // Words of a bit-packed level of side Ll (row-major bits).
 __host__ __device__ constexpr uint32_t level_words(uint32_t Ll) { return (Ll * Ll + 31u) / 32u; }
 // Word offset of level l >= 1 of a side-L lattice in its level buffer.

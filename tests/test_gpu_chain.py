@@ -87,6 +87,19 @@ def test_measurement_records_bit_exact(amd, ctx, orc, L, nl, J):
         assert ch.state() == (n_therm + n_meas, n_meas)
 
 
+def test_l16384_strip_sweeps_bit_exact(amd, ctx, orc):
+    """L = 16384 (256 MiB of int8 spins) updates strips of 8 rows per thread;
+    two sweeps equal the oracle's."""
+    L, J, seed = 16384, 0.4406868, 99
+    _say("L=16384 two sweeps")
+    s = orc.init_spins(L, seed)
+    with amd.Chain(ctx, L, J, seed=seed) as ch:
+        for t in range(2):
+            orc.sweep(L, s, seed, J, t)
+        ch.sweep(2)
+        assert np.array_equal(ch.spins(), s)
+
+
 @pytest.mark.parametrize("L,nl,J", [(512, 8, KC), (1024, 5, 0.4688157), (2048, 5, KC), (4096, 6, 0.4320459)])
 def test_large_lattice_records_bit_exact(amd, ctx, orc, L, nl, J):
     """Levels of side >= 512 are measured by grid launches, the rest by one
