@@ -131,16 +131,21 @@ def run_plain_mixed(mcrg_amd, ctx, jobs, sweeps):
     return updates / (dt * 1e9), dt
 
 
-def run_plain_2048(mcrg_amd, ctx, steps):
-    L, J, seed, nl = 2048, 0.4406868, 7, 5
-    with mcrg_amd.Chain(ctx, L, J, seed, n_levels=nl, sweeps_per_bin=steps + 20, max_bins=1) as ch:
-        ch.sweep(20, measure=True)
+def run_plain(mcrg_amd, ctx, L, nl, steps, measure=True, warm=36):
+    """One plain int8 chain (|K| = 0.4406868, seed 7): `warm` sweeps (which
+    also capture the replayed graph), then `steps` timed sweeps."""
+    with mcrg_amd.Chain(ctx, L, 0.4406868, 7, n_levels=nl, sweeps_per_bin=steps + warm, max_bins=1) as ch:
+        ch.sweep(warm, measure=measure)
         ch.sync()
         t0 = time.perf_counter()
-        ch.sweep(steps, measure=True)
+        ch.sweep(steps, measure=measure)
         ch.sync()
         dt = time.perf_counter() - t0
     return L * L * steps / (dt * 1e9)
+
+
+def run_plain_2048(mcrg_amd, ctx, steps):
+    return run_plain(mcrg_amd, ctx, 2048, 5, steps, True, 20)
 
 
 def run_serving(mcrg_amd, workloads, ctx, rate, duration, drain_limit, quantum, threads):
@@ -294,7 +299,10 @@ def main():
     ap.add_argument("--min-team", type=int, default=16, help="--only batched: lanes per lattice at least")
     ap.add_argument("--max-slot-lds", type=int, default=0, help="--only batched: LDS cap per slot (0 = default)")
     ap.add_argument("--threads-large", type=int, default=0, help="--only batched: workgroup size for L = 512")
-    ap.add_argument("--only", choices=["batched", "plain_l2048"], default=None,
+    ap.add_argument("--plain-L", type=int, default=16384, help="--only plain: lattice side")
+    ap.add_argument("--plain-levels", type=int, default=1, help="--only plain: blockings")
+    ap.add_argument("--plain-measure", type=int, default=0, help="--only plain: measure every sweep (1) or not (0)")
+    ap.add_argument("--only", choices=["batched", "plain_l2048", "plain"], default=None,
                     help="run one leg alone and print its line to stderr (for profiler runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the counter-calibration copy kernel and exit")
@@ -325,6 +333,12 @@ def main():
         value, dt, _, _ = run_batched(mcrg_amd, ctx, jobs, args.steps, args.warmup, args.sweeps_per_step,
                                       args.threads, cfg=cfg)
         log(f"bench: batched {value:.1f} updates/ns ({dt:.3f} s timed), {cfg}")
+        ctx.close()
+        return 0
+    if args.only == "plain":
+        rate = run_plain(mcrg_amd, ctx, args.plain_L, args.plain_levels, args.steps, bool(args.plain_measure))
+        log(f"bench: plain L={args.plain_L} levels={args.plain_levels} measure={args.plain_measure}: "
+            f"{rate:.1f} updates/ns, {args.plain_L ** 2 / rate / 1e3:.1f} us per sweep")
         ctx.close()
         return 0
     if args.only == "plain_l2048":
