@@ -100,10 +100,11 @@ def test_l16384_strip_sweeps_bit_exact(amd, ctx, orc):
         assert np.array_equal(ch.spins(), s)
 
 
-@pytest.mark.parametrize("L,nl,J", [(512, 8, KC), (1024, 5, 0.4688157), (2048, 5, KC), (4096, 6, 0.4320459)])
+@pytest.mark.parametrize("L,nl,J", [(512, 8, KC), (1024, 5, 0.4688157), (2048, 5, KC), (4096, 6, 0.4320459),
+                                    (8192, 7, KC)])
 def test_large_lattice_records_bit_exact(amd, ctx, orc, L, nl, J):
     """Levels of side >= 512 are measured by grid launches, the rest by one
-    workgroup: L = 1024 has one grid level, 2048 two, 4096 three."""
+    workgroup: L = 1024 has one grid level, 2048 two, 4096 three, 8192 four."""
     seed = 7 + L
     n_therm, n_meas, spb = 2, 5, 2
     _say("records (large)", L, nl)
