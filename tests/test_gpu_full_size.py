@@ -1,5 +1,7 @@
-"""The batching configuration at its full size, checked through properties
-that do not need the oracle (which would take hours on it): 4096 replicas of
+"""Configurations at their full size, checked through properties that do not
+need the oracle (which would take hours on them).
+
+Batching: 4096 replicas of
 mcrg_amd.workloads.batch_manifest (L from {8..128}, five couplings, seeds
 0..4095), 5e3 thermalisation and 5e4 measured sweeps each, bins of 1e3.
 
@@ -63,3 +65,25 @@ def test_batching_config_full_size(amd, ctx):
             exact = nn_mean(L, K) / (L * L)
             print(f"L={L} K={K}: {len(sel)} replicas, NN/site {mean:.6f} +- {err:.6f}, exact {exact:.6f}", flush=True)
             assert abs(mean - exact) < 5 * err, (L, K, mean, err, exact)
+
+
+def test_plain_config_l2048_full_size(amd, ctx):
+    """The plain single-lattice configuration at full size: L = 2048,
+    |K| = 0.4406868, seed 7, 2e3 thermalisation and 2e4 measured sweeps, five
+    blockings.  The chain entry point (graph replays of 32 sweeps) and the same
+    job inside a batch (an HBM-schedule group advanced in launches of 250
+    sweeps) give bit-identical spins and records; every bin counts its samples
+    (the first 100 sweeps are checked against the oracle in test_gpu_chain)."""
+    L, J, seed, nl, spb = 2048, 0.4406868, 7, 5, 1000
+    print("plain configuration L=2048: 2e3 + 2e4 sweeps, chain and batch", flush=True)
+    with amd.Chain(ctx, L, J, seed, n_levels=nl, sweeps_per_bin=spb, max_bins=20) as ch:
+        ch.sweep(2000)
+        ch.sweep(20000, measure=True)
+        rec_chain, spins_chain = ch.records(), ch.spins()
+    assert rec_chain.shape == (20, amd.record_size(nl))
+    assert np.all(rec_chain[:, -1] == spb)
+    job = dict(L=L, coupling=J, seed=seed, n_therm=2000, n_meas=20000, n_levels=nl, sweeps_per_bin=spb)
+    with amd.Batch(ctx, [job]) as b:
+        b.run_all()
+        assert np.array_equal(b.records(0), rec_chain)
+        assert np.array_equal(b.spins(0), spins_chain)
