@@ -7,6 +7,8 @@
 //                   measurements per slot with the lattices held in LDS
 //                   (src/ising.cpp:87-149 replaced by Metropolis; src/lattice.cpp:102-120,
 //                   src/mcrg.cpp:72-98 and :314-348 fused)
+#include <mutex>
+
 #include "resident_core.hpp"
 
 namespace mcrg {
@@ -75,13 +77,19 @@ static hipError_t launch_nt(const Slot *slots, uint32_t n_slots, const int32_t *
                             PackedLattice *lats, uint32_t *queue, uint32_t lds_bytes, uint32_t grid,
                             hipStream_t s)
 {
-    // above 64 KiB of dynamic LDS the kernel has to be opted in
+    // Above 64 KiB of dynamic LDS the kernel has to be opted in.  Batches and
+    // servers may launch from different threads: the level only ever rises,
+    // under a lock, so a launch never finds it lowered by another.
+    static std::mutex mu;
     static uint32_t opted_in = 65536u;
-    if (lds_bytes > opted_in) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_resident<NT>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-        if (e != hipSuccess) return e;
-        opted_in = lds_bytes;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (lds_bytes > opted_in) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_resident<NT>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+            if (e != hipSuccess) return e;
+            opted_in = lds_bytes;
+        }
     }
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, queue);
     hipLaunchKernelGGL(k_resident<NT>, dim3(grid), dim3(NT), lds_bytes, s, slots, n_slots, slot_lats,
