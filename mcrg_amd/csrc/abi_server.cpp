@@ -7,6 +7,7 @@
 //           budgets have room
 //   launch  one time slice for every resident job, then wait for it
 // Submitting, polling and reading results only touch the host job table.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -321,12 +322,17 @@ mcrg_status mcrg_server_submit(mcrg_server s, const mcrg_job_desc *job, uint64_t
         set_error("job needs more device memory than the whole budget");
         return MCRG_ERR_OUT_OF_BUDGET;
     }
-    if ((int64_t)s->engine.lds_share(spec.L, spec.n_levels) > s->lds_budget) {
-        set_error("job needs more LDS than the whole LDS budget");
-        return MCRG_ERR_OUT_OF_BUDGET;
-    }
     {
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::lock_guard<std::mutex> lk(s->mu);  // the engine's options change in server_step
+        // the job has to fit the budget on every workgroup size the server may use
+        uint32_t share = s->engine.lds_share(spec.L, spec.n_levels);
+        if (s->adaptive_large && spec.L >= s->eo.large_min_L)
+            share = std::max(s->engine.lds_share(spec.L, spec.n_levels, 512),
+                             s->engine.lds_share(spec.L, spec.n_levels, 1024));
+        if ((int64_t)share > s->lds_budget) {
+            set_error("job needs more LDS than the whole LDS budget");
+            return MCRG_ERR_OUT_OF_BUDGET;
+        }
         if (!s->fatal.empty()) {
             set_error("server stopped after a device error: " + s->fatal);
             return MCRG_ERR_DEVICE;

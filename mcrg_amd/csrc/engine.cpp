@@ -137,9 +137,9 @@ size_t ResidentEngine::job_bytes(const JobSpec &j)
     return spins + rec;
 }
 
-uint32_t ResidentEngine::lds_bytes(int L, int G, int nl) const
+uint32_t ResidentEngine::lds_bytes(int L, int G, int nl, int NT) const
 {
-    const uint32_t T = (uint32_t)threads_for(L) / (uint32_t)G;
+    const uint32_t T = (uint32_t)NT / (uint32_t)G;
     return slot_layout((uint32_t)L, (uint32_t)G, (uint32_t)nl, T).total * 4u;
 }
 
@@ -159,10 +159,9 @@ void ResidentEngine::enable_autotune()
     tune_next_ = 0;
 }
 
-int ResidentEngine::group_size(int L, int nl) const
+int ResidentEngine::group_size(int L, int nl, int NT) const
 {
     const int W = (int)cb_words((uint32_t)L);
-    const int NT = threads_for(L);
     int G = W >= NT ? 1 : NT / W;
     if (opt_.max_team > 0) G = std::max(G, NT / opt_.max_team);
     int cap = 1;
@@ -176,14 +175,14 @@ int ResidentEngine::group_size(int L, int nl) const
     G = std::min(G, std::max(1, NT / team_min));
     // keep a slot's LDS modest so that several workgroups share a CU
     const uint32_t soft = opt_.max_slot_lds ? opt_.max_slot_lds : 32768u * (uint32_t)NT / 256u;
-    while (G > 1 && lds_bytes(L, G, nl) > soft) G /= 2;
+    while (G > 1 && lds_bytes(L, G, nl, NT) > soft) G /= 2;
     return std::max(G, 1);
 }
 
-uint32_t ResidentEngine::lds_share(int L, int nl) const
+uint32_t ResidentEngine::lds_share(int L, int nl, int NT) const
 {
-    const int G = group_size(L, nl);
-    return (lds_bytes(L, G, nl) + (uint32_t)G - 1u) / (uint32_t)G;
+    const int G = group_size(L, nl, NT);
+    return (lds_bytes(L, G, nl, NT) + (uint32_t)G - 1u) / (uint32_t)G;
 }
 
 uint32_t ResidentEngine::slot_sweeps(int L, int G, uint32_t chunk) const
@@ -329,7 +328,7 @@ void ResidentEngine::rebuild_slots(uint32_t chunk)
     slot_lats_.clear();
     for (auto &kv : groups) {
         const int L = kv.first.L, nl = kv.first.nl;
-        const int G = group_size(L, nl);
+        const int G = group_size(L, nl, threads_for(L));
         const std::vector<int> &ids = kv.second;
         for (size_t a = 0; a < ids.size(); a += (size_t)G) {
             const int g = (int)std::min<size_t>((size_t)G, ids.size() - a);
@@ -349,7 +348,7 @@ void ResidentEngine::rebuild_slots(uint32_t chunk)
                 const Job &j = jobs_[(size_t)id];
                 work += (uint64_t)L * (uint64_t)L * std::min<uint64_t>(ns, j.therm_left + j.meas_left);
             }
-            pend.push_back({s, work, lds_bytes(L, gp, nl), threads_for(L)});
+            pend.push_back({s, work, lds_bytes(L, gp, nl, threads_for(L)), threads_for(L)});
         }
     }
     // launch classes by (workgroup size, workgroups resident on the device):

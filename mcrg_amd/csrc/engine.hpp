@@ -120,15 +120,18 @@ class ResidentEngine {
             slots_dirty_ = true;
         }
     }
-    int group_size(int L, int nl) const;
+    // Lattices per slot of side L on NT-thread workgroups.
+    int group_size(int L, int nl, int NT) const;
     // Workgroup size of the slots of side L.
     int threads_for(int L) const
     {
         return opt_.threads_large > 0 && L >= opt_.large_min_L ? opt_.threads_large : opt_.threads;
     }
-    uint32_t lds_bytes(int L, int G, int nl) const;
-    // LDS bytes one lattice of this shape takes in its slot (slot LDS / G).
-    uint32_t lds_share(int L, int nl) const;
+    uint32_t lds_bytes(int L, int G, int nl, int NT) const;
+    // LDS bytes one lattice of this shape takes in its slot (slot LDS / G), on
+    // NT-thread workgroups or on the ones the engine uses now.
+    uint32_t lds_share(int L, int nl, int NT) const;
+    uint32_t lds_share(int L, int nl) const { return lds_share(L, nl, threads_for(L)); }
 
   private:
     struct Job {
