@@ -14,6 +14,9 @@ updates / ns over the timed steps.
 Measured beside it (same JSON line):
   plain_mixed  the same replicas run one at a time through the plain int8 chain
                path (one launch per half-sweep), on a sample of the manifest
+  matched_yt_error  the plain-path sample again through the batched schedule:
+               identical records, hence identical y_t and jackknife error at
+               equal sweeps (the comparison is at matched y_t error)
   plain_l2048  the plain path on one L = 2048 lattice, 5 blockings
   roofline     the resident kernel's algorithmic bytes per launch over its
                HIP-event-timed average duration (events on its own stream)
@@ -118,17 +121,32 @@ def run_batched_timed(mcrg_amd, ctx, jobs, sps, tuned, launches):
 
 
 def run_plain_mixed(mcrg_amd, ctx, jobs, sweeps):
-    """Plain path, one replica at a time: updates/ns over the whole sample."""
+    """Plain path, one replica at a time: updates/ns over the whole sample, and
+    each replica's records."""
     updates = 0
+    recs = []
     t0 = time.perf_counter()
     for j in jobs:
         with mcrg_amd.Chain(ctx, j["L"], j["coupling"], j["seed"], n_levels=j["n_levels"],
                             sweeps_per_bin=sweeps, max_bins=1) as ch:
             ch.sweep(sweeps, measure=True)
             ch.sync()
+            recs.append(ch.records())
         updates += j["L"] * j["L"] * sweeps
     dt = time.perf_counter() - t0
-    return updates / (dt * 1e9), dt
+    return updates / (dt * 1e9), dt, recs
+
+
+def matched_yt(mcrg_amd, ctx, jobs, sweeps, plain_recs):
+    """The same replicas and sweeps through the batched schedule: the records
+    (every sum and product the y_t estimate and its jackknife error are formed
+    from) equal the plain path's bit for bit, so both paths reach the same y_t
+    error after the same sweeps and updates/ns compare at matched error."""
+    bjobs = [dict(j, n_therm=0, n_meas=sweeps, sweeps_per_bin=sweeps) for j in jobs]
+    with mcrg_amd.Batch(ctx, bjobs) as b:
+        b.run_all()
+        same = all(bool((b.records(k) == r).all()) for k, r in enumerate(plain_recs))
+    return {"jobs": len(jobs), "sweeps": sweeps, "records_identical": same}
 
 
 def run_plain(mcrg_amd, ctx, L, nl, steps, measure=True, warm=36):
@@ -361,11 +379,13 @@ def main():
     achieved = bpl / (avg_ms * 1e-3) / 1e9
     log(f"bench: resident kernel {avg_ms:.3f} ms/launch over {n_launch} launches -> {achieved:.1f} GB/s")
 
-    plain_mixed = plain_2048 = None
+    plain_mixed = plain_2048 = matched = None
     if not args.no_plain:
         sample = jobs[:args.plain_sample]
-        plain_mixed, pdt = run_plain_mixed(mcrg_amd, ctx, sample, 20)
+        plain_mixed, pdt, plain_recs = run_plain_mixed(mcrg_amd, ctx, sample, 20)
         log(f"bench: plain path on {len(sample)} replicas x 20 sweeps: {plain_mixed:.3f} updates/ns")
+        matched = matched_yt(mcrg_amd, ctx, sample, 20, plain_recs)
+        log(f"bench: batched records of the same replicas identical: {matched['records_identical']}")
         plain_2048 = run_plain_2048(mcrg_amd, ctx, 500)
         log(f"bench: plain L=2048: {plain_2048:.1f} updates/ns")
     serving = []
@@ -411,6 +431,7 @@ def main():
         "plain_mixed": plain_mixed,
         "plain_l2048": plain_2048,
         "batched_over_plain_mixed": (value / plain_mixed) if plain_mixed else None,
+        "matched_yt_error": matched,
         "serving": serving,
         "bisection": bisection,
         "cpu_baseline": cpu,
