@@ -324,6 +324,27 @@ typedef struct {
 mcrg_status mcrg_rg_exponent(const int64_t *records, int32_t n_bins, int32_t n_levels, int32_t level,
                              int32_t n_ops, mcrg_rg_result *out);
 
+typedef struct {
+    double mean;         /* per-sample mean of the observable                    */
+    double var;          /* per-sample variance                                  */
+    double tau_int;      /* integrated autocorrelation time in samples (sweeps),
+                            from the coarsest binning that keeps >= 32 bins      */
+    double tau_err;      /* its statistical error, tau_int * sqrt(2 / (bins - 1)) */
+    double tau_fine;     /* the same from the bins as given: close to tau_int
+                            when the bins are much longer than tau_int          */
+    int32_t bin_samples; /* samples per bin behind tau_int                      */
+    int32_t n_bins;      /* bins behind tau_int                                 */
+} mcrg_tau_result;
+
+/* Integrated autocorrelation time by binning, from n_bins per-bin records of
+ * equal sample count (a shorter last bin is left out): with B samples per bin,
+ * 2 tau_int = B var(bin means) / var(samples), tau_int = 1/2 + sum_t>=1 rho(t).
+ * op 0, 1, 2 = NN, NNN, P at `level`; op 3 = |M| at level 0.  The bins must be
+ * long against tau_int for the y_t jackknife to hold; this checks that.  No
+ * counterpart in the reference, which does not bin. */
+mcrg_status mcrg_tau_int(const int64_t *records, int32_t n_bins, int32_t n_levels, int32_t level, int32_t op,
+                         mcrg_tau_result *out);
+
 /* One matching step of the critical-point locator, approx_critical_point
  * (src/mcrg.cpp:191-310): `rec_large` is the record of an L lattice, `rec_small`
  * that of an L/2 lattice at the same coupling; at `level` n the L lattice blocked

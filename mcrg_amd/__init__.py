@@ -64,6 +64,12 @@ class RGResult(ctypes.Structure):
                 ("yt_err", ctypes.c_double), ("nu_err", ctypes.c_double), ("n_jackknife", ctypes.c_int32)]
 
 
+class TauResult(ctypes.Structure):
+    _fields_ = [("mean", ctypes.c_double), ("var", ctypes.c_double), ("tau_int", ctypes.c_double),
+                ("tau_err", ctypes.c_double), ("tau_fine", ctypes.c_double), ("bin_samples", ctypes.c_int32),
+                ("n_bins", ctypes.c_int32)]
+
+
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("lds_budget_bytes", ctypes.c_int64),
                 ("launch_quantum", ctypes.c_int64), ("max_jobs", ctypes.c_int32), ("threads", ctypes.c_int32),
@@ -138,6 +144,7 @@ _SIGS = {
     "mcrg_rg_exponent": (_c.c_int32, [_p, _c.c_int32, _c.c_int32, _c.c_int32, _c.c_int32, _c.POINTER(RGResult)]),
     "mcrg_critical_point_step": (_c.c_int32, [_p, _c.c_int32, _p, _c.c_int32, _c.c_int32,
                                               _c.POINTER(_c.c_double)]),
+    "mcrg_tau_int": (_c.c_int32, [_p, _c.c_int32, _c.c_int32, _c.c_int32, _c.c_int32, _c.POINTER(TauResult)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(_lib, _name)
@@ -203,6 +210,20 @@ def rg_exponent(records, n_levels, level, n_ops=2):
     check(_lib.mcrg_rg_exponent(_ptr(r2), r2.shape[0], n_levels, level, n_ops, ctypes.byref(out)))
     return {"lambda": out.lambda_, "yt": out.yt, "nu": out.nu, "yt_err": out.yt_err, "nu_err": out.nu_err,
             "n_jackknife": out.n_jackknife}
+
+
+OP_NN, OP_NNN, OP_P, OP_ABS_M = 0, 1, 2, 3
+
+
+def tau_int(records, n_levels, level=0, op=OP_NN):
+    """Integrated autocorrelation time (in samples) of an operator from per-bin
+    records, by binning: dict(mean, var, tau_int, tau_err, tau_fine,
+    bin_samples, n_bins)."""
+    r = np.ascontiguousarray(records, dtype=np.int64)
+    r2 = r.reshape(-1, record_size(n_levels))
+    out = TauResult()
+    check(_lib.mcrg_tau_int(_ptr(r2), r2.shape[0], n_levels, level, op, ctypes.byref(out)))
+    return {k: getattr(out, k) for k, _ in TauResult._fields_}
 
 
 def critical_point_step(rec_large, nl_large, rec_small, nl_small, level):

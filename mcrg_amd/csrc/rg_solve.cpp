@@ -3,6 +3,7 @@
 // matching step.  Everything starts from the exact int64 per-bin records; the
 // connected correlators are formed exactly in 128-bit integers before the
 // conversion to double.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -227,6 +228,62 @@ mcrg_status mcrg_rg_exponent(const int64_t *records, int32_t n_bins, int32_t n_l
         }
         out->n_jackknife = used;
     }
+    return MCRG_OK;
+}
+
+mcrg_status mcrg_tau_int(const int64_t *records, int32_t n_bins, int32_t n_levels, int32_t level, int32_t op,
+                         mcrg_tau_result *out)
+{
+    MCRG_REQUIRE(records != nullptr && out != nullptr, "null argument");
+    MCRG_REQUIRE(n_levels >= 0 && n_levels <= (int32_t)kMaxLevels, "n_levels out of range");
+    MCRG_REQUIRE(op >= 0 && op <= 3, "op must be 0 (NN), 1 (NNN), 2 (P) or 3 (|M|)");
+    MCRG_REQUIRE(op == 3 ? level == 0 : (level >= 0 && level <= n_levels), "level out of range");
+    const uint32_t nl = (uint32_t)n_levels, rs = record_size(nl);
+    const uint32_t i_sum = op == 3 ? off_M(nl) + 1u : off_S((uint32_t)level, (uint32_t)op);
+    const uint32_t i_sq = op == 3 ? off_M(nl) + 2u : off_self(nl, (uint32_t)level, (uint32_t)op, (uint32_t)op);
+    const uint32_t i_n = off_M(nl) + 3u;
+    // full bins: the sample count of the first bin
+    const int64_t c = n_bins >= 1 ? records[i_n] : 0;
+    int32_t nb = 0;
+    while (nb < n_bins && records[(size_t)nb * rs + i_n] == c) ++nb;
+    MCRG_REQUIRE(c >= 1 && nb >= 2, "need at least two bins of equal, nonzero sample count");
+    i128 s = 0, q = 0;
+    std::vector<double> m((size_t)nb);
+    for (int32_t b = 0; b < nb; ++b) {
+        const int64_t *rb = records + (size_t)b * rs;
+        s += rb[i_sum];
+        q += rb[i_sq];
+        m[(size_t)b] = (double)rb[i_sum] / (double)c;
+    }
+    const i128 n = (i128)c * nb;
+    // per-sample variance from exact sums: (n q - s^2) / n^2
+    const double var = (double)(n * q - s * s) / ((double)n * (double)n);
+    MCRG_REQUIRE(var > 0.0, "the observable does not fluctuate");
+    auto tau_of = [&](const std::vector<double> &mb, double B) {
+        double mean = 0.0;
+        for (double x : mb) mean += x;
+        mean /= (double)mb.size();
+        double v = 0.0;
+        for (double x : mb) v += (x - mean) * (x - mean);
+        v /= (double)(mb.size() - 1);
+        return 0.5 * B * v / var;
+    };
+    std::memset(out, 0, sizeof(*out));
+    out->mean = (double)s / (double)n;
+    out->var = var;
+    out->tau_fine = tau_of(m, (double)c);
+    // merge neighbouring bins while 32 or more remain
+    double B = (double)c;
+    while (m.size() / 2 >= 32) {
+        std::vector<double> mm(m.size() / 2);
+        for (size_t i = 0; i < mm.size(); ++i) mm[i] = 0.5 * (m[2 * i] + m[2 * i + 1]);
+        m.swap(mm);
+        B *= 2.0;
+    }
+    out->tau_int = tau_of(m, B);
+    out->tau_err = out->tau_int * std::sqrt(2.0 / (double)(m.size() - 1));
+    out->bin_samples = (int32_t)std::min<double>(B, 2147483647.0);
+    out->n_bins = (int32_t)m.size();
     return MCRG_OK;
 }
 

@@ -557,14 +557,21 @@ void MonteCarloRenormalizationGroup::calc_critical_exponent(int n_samples_eq, in
     }
     std::fclose(fptr_);
     fptr_ = nullptr;
-    // jackknife errors beside the reference-format file
+    // jackknife errors beside the reference-format file, with the integrated
+    // autocorrelation time of the NN operator at each level (binning): the
+    // jackknife holds while the bins are much longer than it
     FILE *fe = std::fopen((name.substr(0, name.size() - 4) + ".err.txt").c_str(), "w");
     if (fe) {
         std::fprintf(fe, "# delete-one-bin jackknife over %i bins, %i operators\n", n_bins, n_ops_);
-        std::fprintf(fe, "# %23s  %25s  %25s  %25s\n", "Blocking Level n", "nu", "nu error", "y_t error");
-        for (int n = 0; n < n_t; ++n)
-            std::fprintf(fe, "%25i, %25.10lf, %25.10lf, %25.10lf\n", n, results_[(size_t)n].nu,
-                         results_[(size_t)n].nu_err, results_[(size_t)n].yt_err);
+        std::fprintf(fe, "# tau_int: integrated autocorrelation time of NN at level n, in sweeps, by binning\n");
+        std::fprintf(fe, "# %23s  %25s  %25s  %25s  %25s\n", "Blocking Level n", "nu", "nu error", "y_t error",
+                     "tau_int NN");
+        for (int n = 0; n < n_t; ++n) {
+            mcrg_tau_result tr{};
+            const bool have_tau = n_bins >= 2 && mcrg_tau_int(rec.data(), n_bins, n_t, n, 0, &tr) == MCRG_OK;
+            std::fprintf(fe, "%25i, %25.10lf, %25.10lf, %25.10lf, %25.4lf\n", n, results_[(size_t)n].nu,
+                         results_[(size_t)n].nu_err, results_[(size_t)n].yt_err, have_tau ? tr.tau_int : NAN);
+        }
         std::fclose(fe);
     }
     if (!quiet_) std::printf("\n* Critical exponent: nu = %lf\n", nu);

@@ -81,6 +81,10 @@ def test_critical_exponent_file_and_batch_parity(tmp_path):
         assert row[2] == pytest.approx(r["nu"], abs=1e-9)
     err = _read_rows(tmp_path / "critical_exponent_N_16_K_-0.4406868.err.txt")
     assert len(err) == 3 and all(e[2] > 0 for e in err)
+    # the last column is the binning tau_int of NN per level, as mcrg_tau_int gives it
+    for lvl, e in enumerate(err):
+        t = mcrg_amd.tau_int(rec, 3, lvl, mcrg_amd.OP_NN)
+        assert e[4] == pytest.approx(t["tau_int"], abs=1e-4) and t["tau_int"] > 0
 
 
 def test_replicas_and_hbm_schedule(tmp_path):

@@ -64,6 +64,14 @@ def _metro16(amd, ctx):
     return _RUNS["metro16"]
 
 
+def _print_tau(amd, recs, nl, what):
+    """Binning tau_int of NN per level over the replicas' bins (one bin each)."""
+    for lvl in range(nl + 1):
+        t = amd.tau_int(recs, nl, lvl, amd.OP_NN)
+        print(f"{what}: tau_int(NN, level {lvl}) = {t['tau_int']:.2f} +- {t['tau_err']:.2f} sweeps "
+              f"(bins of {t['bin_samples']} samples)", flush=True)
+
+
 def _nu(amd, recs, groups):
     # independent replicas summed into jackknife groups
     grp = recs.reshape(groups, recs.shape[0] // groups, -1).sum(axis=1)
@@ -83,12 +91,14 @@ def _check_nn(amd, recs, L, J, nl, n_sigma=5.0):
 
 def test_nn_swendsen_wang_matches_kaufman(amd, ctx):
     _check_nn(amd, _sw16(amd, ctx), 16, KC, 3)
+    _print_tau(amd, _sw16(amd, ctx), 3, "swendsen-wang L=16")
     print("physics: swendsen-wang L=32 J=0.42", flush=True)
     _check_nn(amd, _replicas(amd, ctx, amd.SAMPLER_SWENDSEN_WANG, 512, 500, 5000, L=32, J=0.42, nl=1), 32, 0.42, 1)
 
 
 def test_nn_metropolis_batched_matches_kaufman(amd, ctx):
     _check_nn(amd, _metro16(amd, ctx), 16, KC, 3)
+    _print_tau(amd, _metro16(amd, ctx), 3, "metropolis L=16")
     print("physics: metropolis L=32 J=0.46", flush=True)
     _check_nn(amd, _replicas(amd, ctx, amd.SAMPLER_METROPOLIS, 512, 5000, 20000, L=32, J=0.46, nl=1), 32, 0.46, 1)
 

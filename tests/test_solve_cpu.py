@@ -107,3 +107,51 @@ def test_critical_point_step_known_answer(amd, orc):
     from test_oracle import kc_known_answer_records
     rl, rs = kc_known_answer_records(orc)
     assert amd.critical_point_step(rl, 2, rs, 1, 0) == pytest.approx(1.0 / 6.0, rel=1e-12)
+
+
+def _ar1_records(rho, n, bin_len, seed, shift=0.0):
+    """Per-bin records (no blocking level) of an AR(1) series x_t = rho x_{t-1}
+    + e_t, scaled by 1000 and rounded, in the NN slot and in the |M| slots."""
+    from scipy.signal import lfilter
+    rng = np.random.default_rng(seed)
+    e = rng.standard_normal(n + 2000)
+    x = lfilter([1.0], [1.0, -rho], e)[2000:]
+    xi = np.rint(1000.0 * (x + shift)).astype(np.int64).reshape(-1, bin_len)
+    rs = 16  # record_size(0)
+    rec = np.zeros((xi.shape[0], rs), dtype=np.int64)
+    rec[:, 0] = xi.sum(axis=1)          # S[0][NN]
+    rec[:, 3] = (xi * xi).sum(axis=1)   # SELF[0][NN][NN]
+    rec[:, 13] = np.abs(xi).sum(axis=1)  # |M|
+    rec[:, 14] = (xi * xi).sum(axis=1)   # M^2
+    rec[:, 15] = bin_len
+    return rec
+
+
+@pytest.mark.parametrize("rho", [0.0, 0.5, 0.9])
+def test_tau_int_ar1_closed_form(amd, rho):
+    """tau_int = (1 + rho) / (2 (1 - rho)) for an AR(1) series (1/2 for white
+    noise); the estimate from given bins of 1024 samples is within 5 %, the
+    coarse one within 4 of its own errors."""
+    exact = (1.0 + rho) / (2.0 * (1.0 - rho))
+    rec = _ar1_records(rho, 1 << 22, 1024, seed=7)
+    for op, shift in ((amd.OP_NN, 0.0), (amd.OP_ABS_M, 50.0)):
+        r = amd.tau_int(_ar1_records(rho, 1 << 22, 1024, seed=7, shift=shift) if shift else rec, 0, 0, op)
+        assert r["tau_fine"] == pytest.approx(exact, rel=0.05), (op, r)
+        assert abs(r["tau_int"] - exact) < 4 * r["tau_err"], (op, r)
+        assert r["n_bins"] == 32 and r["bin_samples"] == (1 << 22) // 32
+    # a planted factor-of-two mistake in the series' correlation would be seen
+    assert abs(amd.tau_int(rec, 0, 0, amd.OP_NN)["tau_fine"] - 2 * exact) > 0.2 * exact
+
+
+def test_tau_int_partial_last_bin_and_errors(amd):
+    rec = _ar1_records(0.5, 1 << 16, 512, seed=3)
+    full = amd.tau_int(rec, 0, 0, amd.OP_NN)
+    short = rec.copy()
+    short[-1, 15] = 100  # a shorter last bin is left out
+    r = amd.tau_int(short, 0, 0, amd.OP_NN)
+    full_minus = amd.tau_int(rec[:-1], 0, 0, amd.OP_NN)
+    assert r == full_minus and r != full
+    with pytest.raises(amd.MCRGError):
+        amd.tau_int(rec[:1], 0, 0, amd.OP_NN)
+    with pytest.raises(amd.MCRGError):
+        amd.tau_int(rec, 0, 0, 4)
