@@ -142,3 +142,28 @@ def test_config_l8_plain_yt_matches_oracle(amd, ctx, orc):
     r2 = amd.rg_exponent(rec2, nl, 0, 2)
     r1 = amd.rg_exponent(rec, nl, 0, 2)
     assert abs(r1["yt"] - r2["yt"]) < 4 * math.hypot(r1["yt_err"], r2["yt_err"])
+
+
+def test_magnetisation_and_yt_agree_between_samplers(amd, ctx):
+    """Swendsen-Wang (a cluster algorithm, like the reference's Wolff sampler)
+    and checkerboard Metropolis sample the same ensemble: at L = 16, K_c the
+    mean |M| and M^2 per site and y_t at every level agree within 5 combined
+    standard errors (replica spread for the moments, jackknife for y_t)."""
+    nl, L = 3, 16
+    off_m = 12 * (nl + 1) + 10 * nl  # M, |M|, M^2, count
+    out = {}
+    for name, recs in (("sw", _sw16(amd, ctx)), ("metropolis", _metro16(amd, ctx))):
+        n = recs[:, off_m + 3].astype(float)
+        absm = recs[:, off_m + 1] / n / (L * L)
+        m2 = recs[:, off_m + 2] / n / float(L * L) ** 2
+        yts = _nu(amd, recs, 64)
+        out[name] = (absm.mean(), absm.std(ddof=1) / math.sqrt(len(absm)),
+                     m2.mean(), m2.std(ddof=1) / math.sqrt(len(m2)), yts)
+        print(f"{name}: <|M|>/N = {absm.mean():.5f} +- {out[name][1]:.5f}, <M^2>/N^2 = {m2.mean():.5f} "
+              f"+- {out[name][3]:.5f}, y_t = " + ", ".join(f"{r['yt']:.4f}({r['yt_err']:.4f})" for r in yts),
+              flush=True)
+    a, b = out["sw"], out["metropolis"]
+    assert abs(a[0] - b[0]) < 5 * math.hypot(a[1], b[1])
+    assert abs(a[2] - b[2]) < 5 * math.hypot(a[3], b[3])
+    for ra, rb in zip(a[4], b[4]):
+        assert abs(ra["yt"] - rb["yt"]) < 5 * math.hypot(ra["yt_err"], rb["yt_err"])
