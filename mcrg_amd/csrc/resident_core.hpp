@@ -421,6 +421,29 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
         }
         if (!measuring) continue;
 
+        // The sample's sums and products into lattice g's partial record,
+        // entries k = k0, k0 + ks, ...: from the operator values in `sv`, or
+        // each entry forming its two values from the counts.
+        auto record_sample = [&](uint32_t g, uint32_t k0, uint32_t ks, bool use_sv) {
+            const uint32_t *m = meta + g * kMetaWords;
+            const uint32_t q = m[kMetaNMeas] + (it - m[kMetaTherm]);
+            const uint32_t spb = m[kMetaSpb];
+            const bool new_bin = q % spb == 0u && q > m[kMetaNMeas];
+            const uint32_t *cntg = cnt + g * cstride;
+            for (uint32_t k = k0; k < rs; k += ks) {
+                int64_t &r = rec[g * rs + k];
+                if (new_bin) {
+                    lats[m[kMetaIdx]].records[(size_t)((q - 1u) / spb) * rs + k] += r;
+                    r = 0;
+                }
+                const uint32_t xy = pairs[k];
+                if (use_sv)
+                    r += (int64_t)sv[g * lay.n_sv + (xy & 0xFFFFu)] * (int64_t)sv[g * lay.n_sv + (xy >> 16)];
+                else
+                    r += sv_value(cntg, nw, nl, L, xy & 0xFFFFu, nl + 1u) * sv_value(cntg, nw, nl, L, xy >> 16, nl + 1u);
+            }
+        };
+
         // ---- level 0: operator counts and blocking into level 1
         for (uint32_t g = tm.g0; g < G; g += tm.gs) {
             const uint32_t *m = meta + g * kMetaWords;
@@ -485,23 +508,31 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
                         pyramid(lvl + g * lay.lvl_words, coins + g * lay.lvl_words, cntg, L, pl, nl, off,
                                 coin_off, u, T < 64u ? T : 64u);
                         if (tm.block) {
-                            // a team spanning waves: its first wave, which holds
-                            // the pyramid's counts, forms the operator values
-                            // (the other partials were published by earlier barriers)
+                            // A team spanning waves: its first wave, which holds
+                            // the pyramid's counts (the other partials were
+                            // published by earlier barriers), forms the operator
+                            // values and updates the record alone.  No barrier
+                            // follows: the other waves go on to the next sweep,
+                            // which touches neither these buffers nor the
+                            // counts before the first wave has met them at the
+                            // end of that sweep's first half.
                             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                             __builtin_amdgcn_wave_barrier();
                             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                             for (uint32_t i = u; i < lay.n_sv; i += 64u)
                                 sv[g * lay.n_sv + i] = (int32_t)sv_value(cntg, nw, nl, L, i, pl);
+                            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                            __builtin_amdgcn_wave_barrier();
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                            record_sample(g, u, 64u, true);
                         }
                     }
                 }
             }
-            team_sync(tm);
+            if (!tm.block) team_sync(tm);
         }
-        // operator values in `sv`: always for teams within a wave, and for
-        // wider teams when the pyramid ran (else each entry forms its own)
-        const bool sv_ready = !tm.block || pl != 0u;
+        // a team spanning waves with a pyramid has updated its record above
+        const bool recorded = tm.block && pl != 0u;
 
         // ---- the sample's sums and products into the partial records.  A
         //      team within one wave first forms each operator value once (its
@@ -518,26 +549,10 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
             }
             team_sync(tm);
         }
-        for (uint32_t g = tm.g0; g < G; g += tm.gs) {
-            const uint32_t *m = meta + g * kMetaWords;
-            if (!in_meas(m, it)) continue;
-            const uint32_t q = m[kMetaNMeas] + (it - m[kMetaTherm]);
-            const uint32_t spb = m[kMetaSpb];
-            const bool new_bin = q % spb == 0u && q > m[kMetaNMeas];
-            const uint32_t *cntg = cnt + g * cstride;
-            for (uint32_t u = tm.u0; u < T; u += tm.us) {
-                for (uint32_t k = u; k < rs; k += T) {
-                    int64_t &r = rec[g * rs + k];
-                    if (new_bin) {
-                        lats[m[kMetaIdx]].records[(size_t)((q - 1u) / spb) * rs + k] += r;
-                        r = 0;
-                    }
-                    const uint32_t xy = pairs[k];
-                    if (sv_ready)
-                        r += (int64_t)sv[g * lay.n_sv + (xy & 0xFFFFu)] * (int64_t)sv[g * lay.n_sv + (xy >> 16)];
-                    else
-                        r += sv_value(cntg, nw, nl, L, xy & 0xFFFFu, nl + 1u) * sv_value(cntg, nw, nl, L, xy >> 16, nl + 1u);
-                }
+        if (!recorded) {
+            for (uint32_t g = tm.g0; g < G; g += tm.gs) {
+                if (!in_meas(meta + g * kMetaWords, it)) continue;
+                for (uint32_t u = tm.u0; u < T; u += tm.us) record_sample(g, u, T, !tm.block);
             }
         }
     }
