@@ -60,16 +60,17 @@ HBM_PEAK_GBS = 8000.0
 # off the same counters on a 1 GiB read + 1 GiB write copy kernel
 # (--copy-calibration).  Configurations without a counter run report null.
 MEASURED_TRAFFIC = {
-    (4096, 100, 256, 16, 0, 0): (3150.9 * 2.0 + 5051.8) * 1024.0,
-    (4096, 100, 256, 16, 65536, 0): (3153.6 * 2.0 + 5051.8) * 1024.0,
+    (4096, 100, 256, 16, 0, 0): (3156.1 * 2.0 + 5047.8) * 1024.0,
+    # two launch classes here (the L = 8 slots grow past 64 KiB): sums over both
+    (4096, 100, 256, 16, 65536, 0): (3343.1 * 2.0 + 5046.9) * 1024.0,
 }
 # The same kernel is bound by VALU issue, not HBM: rocprofv3 --pmc SQ_INSTS_VALU
 # and GRBM_GUI_ACTIVE over the same launches give the wave-level VALU
 # instructions per launch and the busy cycles per XCD (8 XCDs, 1024 SIMDs that
 # issue one wave64 VALU instruction per 4 cycles).
 MEASURED_VALU = {
-    (4096, 100, 256, 16, 0, 0): {"valu_wave_instr_per_launch": 6.212e8, "gui_active_cycles_per_xcd": 33963140.6 / 8},
-    (4096, 100, 256, 16, 65536, 0): {"valu_wave_instr_per_launch": 6.212e8, "gui_active_cycles_per_xcd": 34290596.0 / 8},
+    (4096, 100, 256, 16, 0, 0): {"valu_wave_instr_per_launch": 5.801e8, "gui_active_cycles_per_xcd": 33973861.7 / 8},
+    (4096, 100, 256, 16, 65536, 0): {"valu_wave_instr_per_launch": 5.743e8, "gui_active_cycles_per_xcd": 34382009.0 / 8},
 }
 
 
