@@ -253,7 +253,7 @@ def cpu_baseline(jobs, sweeps, threads):
         updates = sum(ex.map(one, jobs))
     dt = time.perf_counter() - t0
     return {"value": updates / (dt * 1e9), "unit": "spin-updates/ns", "cores": threads, "kind": "port",
-            "sample": f"oracle (scalar C, -O2) on {len(jobs)} replicas of the batching manifest x {sweeps} "
+            "sample": f"oracle (scalar C, -O3) on {len(jobs)} replicas of the batching manifest x {sweeps} "
                       f"measured sweeps each, {threads} threads over independent replicas: {dt:.1f} s"}
 
 
