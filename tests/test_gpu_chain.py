@@ -87,6 +87,19 @@ def test_measurement_records_bit_exact(amd, ctx, orc, L, nl, J):
         assert ch.state() == (n_therm + n_meas, n_meas)
 
 
+def test_l16384_all_levels_record_bit_exact(amd, ctx, orc):
+    """The largest lattice with every blocking (16384 -> 2: five grid-measured
+    levels, then eight in one workgroup): one measured sweep's record equals
+    the oracle's."""
+    L, J, seed, nl = 16384, 0.4406868, 5, 13
+    _say("L=16384 13 levels, one measured sweep")
+    s_ref, rec_ref = orc.run_chain(L, J, seed, 0, 1, nl, 1)
+    with amd.Chain(ctx, L, J, seed=seed, n_levels=nl, sweeps_per_bin=1, max_bins=1) as ch:
+        ch.sweep(1, measure=True)
+        assert np.array_equal(ch.records(), rec_ref)
+        assert np.array_equal(ch.spins(), s_ref)
+
+
 def test_l16384_strip_sweeps_bit_exact(amd, ctx, orc):
     """L = 16384 (256 MiB of int8 spins) updates strips of 8 rows per thread;
     two sweeps equal the oracle's."""
