@@ -27,8 +27,15 @@ struct u32x4 {
     uint32_t v0, v1, v2, v3;
 };
 
+// a ^ b ^ c as one v_bitop3_b32 (gfx950); truth table 0x96 = odd parity of the
+// three inputs.  The compiler emits two v_xor_b32 for the plain expression.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 // Philox4x32-10 (Salmon et al., SC'11).  Ten rounds of two 32x32->64 multiplies;
-// on CDNA each multiply is one v_mad_u64_u32.
+// on CDNA each multiply is one v_mad_u64_u32, each round's key mix one xor3.
 __device__ __forceinline__ u32x4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                         uint32_t k0, uint32_t k1)
 {
@@ -36,8 +43,8 @@ __device__ __forceinline__ u32x4 philox(uint32_t c0, uint32_t c1, uint32_t c2, u
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -154,7 +161,7 @@ __device__ __forceinline__ uint32_t block16(uint32_t w0, uint32_t w1, uint32_t c
     const uint32_t E = 0x55555555u;
     const uint32_t b1 = w0 & E, b2 = (w0 >> 1) & E, b3 = w1 & E, b4 = (w1 >> 1) & E;
     const uint32_t maj3 = (b1 & b2 & (b3 | b4)) | (b3 & b4 & (b1 | b2));
-    const uint32_t par = b1 ^ b2 ^ b3 ^ b4;
+    const uint32_t par = xor3(b1, b2, b3) ^ b4;
     const uint32_t all = b1 & b2 & b3 & b4;
     const uint32_t some = b1 | b2 | b3 | b4;
     const uint32_t tie = ~par & ~all & some & E;

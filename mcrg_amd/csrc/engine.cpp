@@ -530,9 +530,14 @@ hipError_t ResidentEngine::tick(uint32_t n, uint32_t &advanced, KernelTimer *tm)
         if ((e = hipEventElapsedTime(&ms, tune_a_, tune_b_)) != hipSuccess) return e;
         tune_[tune_next_].rate = ms > 0.f ? updates / (double)ms : 0.0;
         if (++tune_next_ == tune_.size()) {
-            size_t best = 1;  // entry 0 only warmed up
-            for (size_t i = 2; i < tune_.size(); ++i)
+            // entry 0 only warmed up; entry 1 (the configuration in use) is
+            // replaced only by a candidate faster by more than one launch's
+            // timing noise (about 1 %), so near-equal candidates do not flip
+            // the choice from run to run
+            size_t best = 2;
+            for (size_t i = 3; i < tune_.size(); ++i)
                 if (tune_[i].rate > tune_[best].rate) best = i;
+            if (tune_[best].rate <= kTuneMargin * tune_[1].rate) best = 1;
             opt_.threads = tune_[best].threads;
             opt_.min_team = std::min(tune_[best].min_team, tune_[best].threads);
             opt_.threads_large = tune_[best].threads_large;

@@ -48,6 +48,10 @@ class Arena {
 constexpr uint64_t kSweepFixedNs = 8000;
 constexpr uint64_t kSweepPerSiteNs = 75;
 
+// The autotuner keeps the configuration in use unless a candidate's measured
+// rate exceeds it by this factor.
+constexpr double kTuneMargin = 1.02;
+
 struct EngineOptions {
     int threads = 256;           // workgroup size of the resident kernel
     int sweeps_per_launch = 250;  // chunk length (the most sweeps any lattice runs per launch)

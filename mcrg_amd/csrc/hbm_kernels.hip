@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void k_int8_measure0(const HbmLattice *__restr
                 const uint32_t rt = (s >> 1) | (nx[r] << 31), g = (d >> 1) | (nx[r + 1] << 31);
                 c[0] += popc(s ^ rt) + popc(s ^ d);
                 c[1] += popc(s ^ g) + popc(rt ^ d);
-                c[2] += popc(s ^ rt ^ d ^ g);
+                c[2] += popc(xor3(s, rt, d) ^ g);
                 c[3] += popc(s);
             }
             // coin word i >> 1 of level 0, its half i & 1 (the half-word's bit offset mod 32)

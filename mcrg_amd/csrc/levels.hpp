@@ -59,7 +59,7 @@ __device__ __forceinline__ void rm_ops_item(const uint32_t *buf, uint32_t Ll, ui
     }
     dnn += (uint32_t)(popc(a ^ r) + popc(a ^ d));
     dnnn += (uint32_t)(popc(a ^ g) + popc(r ^ d));
-    dp += (uint32_t)popc(a ^ r ^ d ^ g);
+    dp += (uint32_t)popc(xor3(a, r, d) ^ g);
 }
 
 // Output words of rm_block_item (level of side Ll -> Ll/2).
@@ -219,7 +219,7 @@ __device__ __forceinline__ void cb_ops_item(const uint32_t *own, const uint32_t 
     if (c == 0u)
         dnn += (uint32_t)(popc(s ^ n.nu) + popc(s ^ n.nd) + popc(s ^ n.nl) + popc(s ^ n.nr));
     dnnn += (uint32_t)(popc(s ^ dr) + popc(s ^ dl));
-    dp += (uint32_t)popc(s ^ n.nr ^ n.nd ^ dr);
+    dp += (uint32_t)popc(xor3(s, n.nr, n.nd) ^ dr);
     ndown += (uint32_t)popc(s);
 }
 
@@ -238,7 +238,7 @@ __device__ __forceinline__ uint32_t cb_block_item(const uint32_t *c0, const uint
         const uint32_t r0 = ((2u * by) << sh) + q, r1 = ((2u * by + 1u) << sh) + q;
         const uint32_t b1 = c0[r0], b2 = c1[r0], b3 = c0[r1], b4 = c1[r1];
         const uint32_t maj3 = (b1 & b2 & (b3 | b4)) | (b3 & b4 & (b1 | b2));
-        const uint32_t tie = ~(b1 ^ b2 ^ b3 ^ b4) & ~(b1 & b2 & b3 & b4) & (b1 | b2 | b3 | b4);
+        const uint32_t tie = ~(xor3(b1, b2, b3) ^ b4) & ~(b1 & b2 & b3 & b4) & (b1 | b2 | b3 | b4);
         return maj3 | (tie & coin);
     }
     // L in {8, 16, 32}: CB word w holds rows w*R .. w*R + R - 1 (R = 64/L, even),
@@ -252,7 +252,7 @@ __device__ __forceinline__ uint32_t cb_block_item(const uint32_t *c0, const uint
         const uint32_t b1 = a0 & m.even, b2 = a1 & m.even;
         const uint32_t b3 = (a0 >> m.r) & m.even, b4 = (a1 >> m.r) & m.even;
         const uint32_t maj3 = (b1 & b2 & (b3 | b4)) | (b3 & b4 & (b1 | b2));
-        const uint32_t tie = ~(b1 ^ b2 ^ b3 ^ b4) & ~(b1 & b2 & b3 & b4) & (b1 | b2 | b3 | b4) & m.even;
+        const uint32_t tie = ~(xor3(b1, b2, b3) ^ b4) & ~(b1 & b2 & b3 & b4) & (b1 | b2 | b3 | b4) & m.even;
         const uint32_t c16 = (coin >> (16u * h)) & 0xFFFFu;
         out |= (compress_seg(maj3, m.r) | (compress_seg(tie, m.r) & c16)) << (16u * h);
     }

@@ -267,16 +267,17 @@ __device__ __forceinline__ void pyramid(const uint32_t *lg, const uint32_t *cg, 
                 const uint32_t dnnn =
                     (uint32_t)(popc(lo ^ glo) + popc(hi ^ ghi) + popc(rlo ^ dlo) + popc(rhi ^ dhi));
                 nn_nnn = dnn | (dnnn << 16);
-                dp = (uint32_t)(popc(lo ^ rlo ^ dlo ^ glo) + popc(hi ^ rhi ^ dhi ^ ghi));
+                dp = (uint32_t)(popc(xor3(lo, rlo, dlo) ^ glo) + popc(xor3(hi, rhi, dhi) ^ ghi));
             } else {
                 const uint32_t r = Ll == 32u ? ((lo >> 1) | (lo << 31)) : rot_right_small(lo, Ll);
                 const uint32_t g = Ll == 32u ? ((dlo >> 1) | (dlo << 31)) : rot_right_small(dlo, Ll);
                 nn_nnn = (uint32_t)(popc(lo ^ r) + popc(lo ^ dlo)) | ((uint32_t)(popc(lo ^ g) + popc(r ^ dlo)) << 16);
-                dp = (uint32_t)popc(lo ^ r ^ dlo ^ g);
+                dp = (uint32_t)popc(xor3(lo, r, dlo) ^ g);
             }
         }
-        nn_nnn = group_sum(nn_nnn, width);
-        dp = group_sum(dp, width);
+        // rows y >= Ll contribute nothing: sum over the level's Ll lanes
+        nn_nnn = group_sum(nn_nnn, Ll);
+        dp = group_sum(dp, Ll);
         if (y == 0u) {
             uint32_t *c = cntg + l * 4u;
             c[0] = nn_nnn & 0xFFFFu;
