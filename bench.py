@@ -59,6 +59,9 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 KiB per dispatch, the scale factors read
 # off the same counters on a 1 GiB read + 1 GiB write copy kernel
 # (--copy-calibration).  Configurations without a counter run report null.
+# Changes to the kernel's arithmetic alone leave these unchanged: across the
+# change of the Philox key mix to one v_bitop3_b32 the default configuration
+# read 3,156.1 then 3,157.0 KiB and wrote 5,047.8 then 5,048 KiB.
 MEASURED_TRAFFIC = {
     (4096, 100, 256, 16, 0, 0): (3156.1 * 2.0 + 5047.8) * 1024.0,
     # two launch classes here (the L = 8 slots grow past 64 KiB): sums over both
@@ -68,9 +71,11 @@ MEASURED_TRAFFIC = {
 # and GRBM_GUI_ACTIVE over the same launches give the wave-level VALU
 # instructions per launch and the busy cycles per XCD (8 XCDs, 1024 SIMDs that
 # issue one wave64 VALU instruction per 4 cycles).
+# Measured on the kernel with the Philox key mix as one v_bitop3_b32; the
+# blocking and plaquette parity terms have since taken the same form and the
+# register pyramid sums fewer lanes, so the current count is at most this.
 MEASURED_VALU = {
-    (4096, 100, 256, 16, 0, 0): {"valu_wave_instr_per_launch": 5.801e8, "gui_active_cycles_per_xcd": 33973861.7 / 8},
-    (4096, 100, 256, 16, 65536, 0): {"valu_wave_instr_per_launch": 5.743e8, "gui_active_cycles_per_xcd": 34382009.0 / 8},
+    (4096, 100, 256, 16, 0, 0): {"valu_wave_instr_per_launch": 5.5006e8, "gui_active_cycles_per_xcd": 32599700.0 / 8},
 }
 
 
