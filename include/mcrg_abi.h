@@ -196,8 +196,10 @@ typedef struct {
                                   units, else 512                               */
     int32_t autotune;          /* nonzero: the first launches each try one
                                   candidate configuration (timed with events)
-                                  and the fastest is kept; results are the same
-                                  under every configuration                     */
+                                  and the fastest is kept if it beats the
+                                  configuration given here by more than 2 %;
+                                  results are the same under every
+                                  configuration                                 */
 } mcrg_batch_opts;
 
 mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n_jobs,
