@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MCRG_ABI_VERSION 1
+#define MCRG_ABI_VERSION 2
 
 typedef int32_t mcrg_status;
 enum {
@@ -200,6 +200,16 @@ typedef struct {
                                   configuration given here by more than 2 %;
                                   results are the same under every
                                   configuration                                 */
+    int32_t sub_chunks;        /* parts each workgroup slot's sweeps of one
+                                  launch are split into (0..64); 0 = automatic:
+                                  up to 4 parts of at least 16 sweeps when the
+                                  slots are more than the compute units but
+                                  fewer than twice the resident workgroups, so
+                                  that no CU holding one slot more than the
+                                  others sets the launch time; else 1.  Parts
+                                  of a slot run in order, possibly in
+                                  different workgroups; results are the same
+                                  for every value                               */
 } mcrg_batch_opts;
 
 mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n_jobs,

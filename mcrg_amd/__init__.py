@@ -56,7 +56,8 @@ class JobDesc(ctypes.Structure):
 class BatchOpts(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int32), ("sweeps_per_launch", ctypes.c_int32),
                 ("max_group", ctypes.c_int32), ("max_slot_lds", ctypes.c_int32), ("min_team", ctypes.c_int32),
-                ("max_team", ctypes.c_int32), ("threads_large", ctypes.c_int32), ("autotune", ctypes.c_int32)]
+                ("max_team", ctypes.c_int32), ("threads_large", ctypes.c_int32), ("autotune", ctypes.c_int32),
+                ("sub_chunks", ctypes.c_int32)]
 
 
 class RGResult(ctypes.Structure):
@@ -344,13 +345,13 @@ class Batch:
     n_therm, n_meas, n_levels, sweeps_per_bin."""
 
     def __init__(self, ctx, jobs, threads=0, sweeps_per_launch=0, max_group=0, max_slot_lds=0, min_team=0,
-                 max_team=0, threads_large=0, autotune=False):
+                 max_team=0, threads_large=0, autotune=False, sub_chunks=0):
         arr = (JobDesc * len(jobs))()
         for i, j in enumerate(jobs):
             arr[i] = JobDesc(j["L"], float(j["coupling"]), j["seed"], j.get("n_therm", 0), j.get("n_meas", 0),
                              j.get("n_levels", 0), j.get("sweeps_per_bin", 1), j.get("sampler", 0))
         opts = BatchOpts(threads, sweeps_per_launch, max_group, max_slot_lds, min_team, max_team, threads_large,
-                         1 if autotune else 0)
+                         1 if autotune else 0, sub_chunks)
         h = ctypes.c_void_p()
         check(_lib.mcrg_batch_create(ctx.handle, arr, len(jobs), ctypes.byref(opts), ctypes.byref(h)))
         self.handle = h

@@ -76,7 +76,9 @@ mcrg_status mcrg_batch_create(mcrg_ctx ctx, const mcrg_job_desc *jobs, int32_t n
         if (opts->min_team) eo.min_team = opts->min_team;
         eo.max_team = opts->max_team;
         eo.threads_large = opts->threads_large;
+        eo.sub_chunks = opts->sub_chunks;
     }
+    MCRG_REQUIRE(eo.sub_chunks >= 0 && eo.sub_chunks <= 64, "sub_chunks must be in [0, 64]");
     MCRG_REQUIRE(eo.threads_large == 0 || eo.threads_large == 64 || eo.threads_large == 128 ||
                      eo.threads_large == 256 || eo.threads_large == 512 || eo.threads_large == 1024,
                  "threads_large must be 0, 64, 128, 256, 512 or 1024");
@@ -289,6 +291,7 @@ mcrg_status mcrg_batch_get_opts(mcrg_batch b, mcrg_batch_opts *out)
     out->max_team = o.max_team;
     out->threads_large = o.threads_large;
     out->autotune = b->engine.tuning() ? 1 : 0;
+    out->sub_chunks = o.sub_chunks;
     return MCRG_OK;
 }
 

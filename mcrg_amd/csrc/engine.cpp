@@ -89,6 +89,8 @@ ResidentEngine::~ResidentEngine()
     if (d_slots_) (void)hipFree(d_slots_);
     if (d_slot_lats_) (void)hipFree(d_slot_lats_);
     if (d_queue_) (void)hipFree(d_queue_);
+    if (d_done_) (void)hipFree(d_done_);
+    if (d_err_) (void)hipFree(d_err_);
     if (d_init_idx_) (void)hipFree(d_init_idx_);
 }
 
@@ -103,6 +105,9 @@ hipError_t ResidentEngine::init(int device, size_t arena_bytes, int capacity, co
     if ((e = arena_.init(arena_bytes)) != hipSuccess) return e;
     if ((e = hipMalloc(&d_lats_, sizeof(PackedLattice) * (size_t)capacity)) != hipSuccess) return e;
     if ((e = hipMalloc(&d_queue_, 64u * kMaxClasses)) != hipSuccess) return e;
+    if ((e = hipMalloc(&d_err_, sizeof(uint32_t))) != hipSuccess) return e;
+    if ((e = hipMemset(d_err_, 0, sizeof(uint32_t))) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) return e;
     if ((e = hipMalloc(&d_init_idx_, sizeof(int32_t) * (size_t)capacity)) != hipSuccess) return e;
     for (hipStream_t &a : aux_)
         if ((e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking)) != hipSuccess) return e;
@@ -426,6 +431,11 @@ hipError_t ResidentEngine::upload_slots()
                            hipMemcpyHostToDevice)) != hipSuccess)
             return e;
     }
+    if (slots_.size() > done_cap_) {
+        if (d_done_) (void)hipFree(d_done_);
+        done_cap_ = std::max<size_t>(slots_.size(), 2 * done_cap_);
+        if ((e = hipMalloc(&d_done_, sizeof(uint32_t) * done_cap_)) != hipSuccess) return e;
+    }
     for (LaunchClass &lc : classes_) {
         const int resident = resident_max_active_groups(lc.threads, lc.lds);
         if (resident <= 0) {
@@ -433,6 +443,28 @@ hipError_t ResidentEngine::upload_slots()
             return hipErrorInvalidConfiguration;
         }
         lc.grid = std::min<int>(resident, (int)lc.count);
+        uint32_t min_sweeps = 0xFFFFFFFFu;
+        for (uint32_t i = lc.first; i < lc.first + lc.count; ++i) min_sweeps = std::min(min_sweeps, slots_[i].n_sweeps);
+        if (opt_.sub_chunks > 0)
+            lc.n_sub = (uint32_t)opt_.sub_chunks;
+        else if ((int)lc.count > cus_ && (int)lc.count < 2 * resident)
+            lc.n_sub = std::max(1u, std::min(4u, min_sweeps / 16u));
+        else
+            lc.n_sub = 1;
+    }
+    return hipSuccess;
+}
+
+hipError_t ResidentEngine::check_parts()
+{
+    if (!parted_) return hipSuccess;
+    uint32_t err = 0;
+    hipError_t e;
+    if ((e = hipMemcpy(&err, d_err_, sizeof(err), hipMemcpyDeviceToHost)) != hipSuccess) return e;
+    parted_ = false;
+    if (err != 0u) {
+        failed_ = "k_resident: a slot part waited for its predecessor beyond the limit";
+        return hipErrorLaunchFailure;
     }
     return hipSuccess;
 }
@@ -449,8 +481,10 @@ hipError_t ResidentEngine::launch_classes(KernelTimer *tm)
         hipStream_t s = c == 0 ? stream_ : aux_[c - 1];
         if (c > 0 && (e = hipStreamWaitEvent(s, ev_fork_, 0)) != hipSuccess) return e;
         // one queue counter per class, 64 bytes apart
-        if ((e = launch_resident(lc.threads, d_slots_ + lc.first, lc.count, d_slot_lats_, d_lats_,
-                                 d_queue_ + 16u * c, lc.lds, (uint32_t)lc.grid, s)) != hipSuccess) {
+        if (lc.n_sub > 1u) parted_ = true;
+        if ((e = launch_resident(lc.threads, d_slots_ + lc.first, lc.count, lc.n_sub, d_slot_lats_, d_lats_,
+                                 d_queue_ + 16u * c, d_done_ + lc.first, d_err_, lc.lds, (uint32_t)lc.grid, s)) !=
+            hipSuccess) {
             failed_ = "launch_resident";
             return e;
         }
@@ -559,6 +593,7 @@ hipError_t ResidentEngine::read_spins(int id, int8_t *host)
                             stream_)) != hipSuccess)
         return e;
     if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
+    if ((e = check_parts()) != hipSuccess) return e;
     for (uint32_t c = 0; c < 2u; ++c)
         for (uint32_t w = 0; w < W; ++w)
             for (uint32_t b = 0; b < 32u; ++b) {
@@ -578,7 +613,8 @@ hipError_t ResidentEngine::read_records(int id, int64_t *host, int64_t n)
     const uint32_t W = cb_words((uint32_t)job.spec.L);
     const char *rec = arena_.base() + job.off + round_up(2u * W * 4u, 256);
     if ((e = hipMemcpyAsync(host, rec, (size_t)n * 8u, hipMemcpyDeviceToHost, stream_)) != hipSuccess) return e;
-    return hipStreamSynchronize(stream_);
+    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
+    return check_parts();
 }
 
 }  // namespace mcrg

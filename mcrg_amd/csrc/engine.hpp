@@ -71,6 +71,14 @@ struct EngineOptions {
     // so that every workgroup works about quantum_ns; 0 = every slot runs the
     // chunk.
     uint64_t quantum_ns = 0;
+    // Parts per slot and launch.  A launch with fewer slots than twice its
+    // resident workgroups and more than one per CU gives each workgroup
+    // exactly one slot, so the CUs that hold one slot more than the others
+    // set the launch time.  Split into sequential parts (queued part-major,
+    // each part waiting for the previous one of its slot), the slots' work
+    // spreads over the workgroups as they free up.  0 = automatic (up to 4
+    // parts of at least 16 sweeps in such launches, else 1).
+    int sub_chunks = 0;
 };
 
 class ResidentEngine {
@@ -166,6 +174,7 @@ class ResidentEngine {
     struct LaunchClass {
         uint32_t first = 0, count = 0, lds = 0;
         int threads = 0, grid = 0;
+        uint32_t n_sub = 1;  // parts per slot (sub_chunks)
     };
     std::vector<LaunchClass> classes_;
     hipStream_t aux_[kMaxClasses - 1] = {};
@@ -189,6 +198,12 @@ class ResidentEngine {
     int32_t *d_slot_lats_ = nullptr;
     size_t slot_cap_ = 0, slot_lat_cap_ = 0;
     uint32_t *d_queue_ = nullptr;
+    uint32_t *d_done_ = nullptr;  // per-slot progress of parted launches
+    size_t done_cap_ = 0;
+    uint32_t *d_err_ = nullptr;   // set by a part whose wait timed out
+    bool parted_ = false;         // a parted launch ran since the last check
+    int cus_ = 0;
+    hipError_t check_parts();
     int32_t *d_init_idx_ = nullptr;
     bool slots_dirty_ = true;
     uint32_t slots_chunk_ = 0;

@@ -81,9 +81,17 @@ __host__ __device__ inline SlotLayout slot_layout(uint32_t L, uint32_t G, uint32
 }
 
 hipError_t launch_packed_init(PackedLattice *lats, const int32_t *idx, int n, hipStream_t s);
-hipError_t launch_resident(int threads, const Slot *slots, uint32_t n_slots, const int32_t *slot_lats,
-                           PackedLattice *lats, uint32_t *queue, uint32_t lds_bytes,
-                           uint32_t grid, hipStream_t s);
+// Work items of one launch: sub-chunk k (0 <= k < n_sub) of slot i is item
+// k * n_slots + i, so that every slot's first part is queued before any second
+// part.  Sub-chunk k runs sweeps [k c, min((k+1) c, n_sweeps)) of its slot,
+// c = ceil(n_sweeps / n_sub), and starts only after sub-chunk k-1 of the same
+// slot has published its write-back (done[i] = k); with n_sub = 1 there is no
+// waiting.  `done` holds n_slots words and `queue` one, both zeroed by the
+// launch; a wait that exceeds kSubChunkWaitTicks sets *err.
+constexpr uint64_t kSubChunkWaitTicks = 2000000000ull;  // 20 s of the 100 MHz clock
+hipError_t launch_resident(int threads, const Slot *slots, uint32_t n_slots, uint32_t n_sub,
+                           const int32_t *slot_lats, PackedLattice *lats, uint32_t *queue, uint32_t *done,
+                           uint32_t *err, uint32_t lds_bytes, uint32_t grid, hipStream_t s);
 int resident_max_active_groups(int threads, uint32_t lds_bytes);
 
 }  // namespace mcrg

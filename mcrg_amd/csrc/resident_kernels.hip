@@ -35,33 +35,97 @@ __global__ __launch_bounds__(256) void k_packed_init(PackedLattice *__restrict__
     P.spins[j] = word;
 }
 
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+// Waits until *flag >= want (agent-scope acquire), polling with s_sleep.
+// False when the 100 MHz real-time clock has advanced by more than
+// kSubChunkWaitTicks first.
+__device__ bool wait_flag(const uint32_t *flag, uint32_t want)
+{
+    const uint64_t t0 = (uint64_t)wall_clock64();
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
+        __builtin_amdgcn_s_sleep(4);
+        if ((uint64_t)wall_clock64() - t0 > kSubChunkWaitTicks) return false;
+    }
+    return true;
+}
+
 template <int NT>
-__global__ __launch_bounds__(NT) void k_resident(const Slot *__restrict__ slots, uint32_t n_slots,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_resident(const Slot *__restrict__ slots, uint32_t n_slots, uint32_t n_sub,
                                                  const int32_t *__restrict__ slot_lats,
-                                                 PackedLattice *__restrict__ lats, uint32_t *queue)
+                                                 PackedLattice *__restrict__ lats, uint32_t *queue, uint32_t *done,
+                                                 uint32_t *err)
 {
     extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_idx;
+    __shared__ uint32_t s_idx, s_ok;
+    const uint32_t n_items = n_slots * n_sub;
+    // The part finished last is published (done[pub_si] = pub_val) in the same
+    // thread-0 block that takes the next item.  A thread-0 store of its own at
+    // the end of the loop body was merged by the compiler with the next
+    // iteration's thread-0 dequeue into a lane-divergent loop, in which the
+    // other lanes of the first wave passed the loop-top barrier before lane 0
+    // had published and dequeued, and the workgroup repeated its item forever.
+    uint32_t pub_si = 0, pub_val = 0;
     for (;;) {
-        if (threadIdx.x == 0) s_idx = atomicAdd(queue, 1u);
+        if (threadIdx.x == 0) {
+            if (pub_val != 0u)
+                __hip_atomic_store(done + pub_si, pub_val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            s_idx = atomicAdd(queue, 1u);
+        }
         __syncthreads();
-        const uint32_t idx = s_idx;
+        // the item and its slot are the same in every lane: keep them scalar
+        const uint32_t idx = uni(s_idx);
         __syncthreads();
-        if (idx >= n_slots) break;
-        const Slot sl = slots[idx];
-        Team tm;
-        tm.T = (uint32_t)NT / (uint32_t)sl.G;
-        tm.g0 = threadIdx.x / tm.T;
-        tm.gs = (uint32_t)sl.G;
-        tm.u0 = threadIdx.x % tm.T;
-        tm.us = tm.T;
-        tm.lane = threadIdx.x;
-        tm.block = tm.T > 64u;
-        run_slot(sl, slot_lats, lats, lds, tm);
+        if (idx >= n_items) break;
+        const uint32_t k = uni(idx / n_slots), si = idx - k * n_slots;
+        Slot sl = slots[si];
+        const uint32_t per = uni((sl.n_sweeps + n_sub - 1u) / n_sub), start = k * per;
+        sl.n_sweeps = start < sl.n_sweeps ? min(per, sl.n_sweeps - start) : 0u;
+        bool ok = true;
+        if (k > 0u) {
+            // the previous part of this slot ran in another workgroup, perhaps
+            // on another XCD: wait for its write-back to be published
+            if (threadIdx.x == 0) {
+                const bool w = wait_flag(done + si, k);
+                if (!w) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_ok = w ? 1u : 0u;
+            }
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            ok = uni(s_ok) != 0u;
+        }
+        if (ok && sl.n_sweeps > 0u) {
+            Team tm;
+            tm.T = (uint32_t)NT / (uint32_t)sl.G;
+            tm.g0 = threadIdx.x / tm.T;
+            tm.gs = (uint32_t)sl.G;
+            tm.u0 = threadIdx.x % tm.T;
+            tm.us = tm.T;
+            tm.lane = threadIdx.x;
+            tm.block = tm.T > 64u;
+            run_slot(sl, slot_lats, lats, lds, tm);
+        }
+        if (n_sub > 1u) {
+            // this part's write-back (spins, records, counters) goes to the
+            // workgroup that runs the next part: released here by every
+            // thread, published at the loop top; after a timed-out wait the
+            // part is skipped and published all the same, so that no wait
+            // behind it lasts (the host reports *err)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            pub_si = si;
+            pub_val = k + 1u;
+        }
     }
 }
 
-__global__ void k_reset_queue(uint32_t *queue) { *queue = 0u; }
+// Zeroes the work counter and the n per-slot progress words of one launch.
+__global__ void k_reset_queue(uint32_t *queue, uint32_t *done, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0u) *queue = 0u;
+    if (i < n) done[i] = 0u;
+}
 
 hipError_t launch_packed_init(PackedLattice *lats, const int32_t *idx, int n, hipStream_t s)
 {
@@ -73,9 +137,9 @@ hipError_t launch_packed_init(PackedLattice *lats, const int32_t *idx, int n, hi
 }
 
 template <int NT>
-static hipError_t launch_nt(const Slot *slots, uint32_t n_slots, const int32_t *slot_lats,
-                            PackedLattice *lats, uint32_t *queue, uint32_t lds_bytes, uint32_t grid,
-                            hipStream_t s)
+static hipError_t launch_nt(const Slot *slots, uint32_t n_slots, uint32_t n_sub, const int32_t *slot_lats,
+                            PackedLattice *lats, uint32_t *queue, uint32_t *done, uint32_t *err,
+                            uint32_t lds_bytes, uint32_t grid, hipStream_t s)
 {
     // Above 64 KiB of dynamic LDS the kernel has to be opted in.  Batches and
     // servers may launch from different threads: the level only ever rises,
@@ -91,22 +155,25 @@ static hipError_t launch_nt(const Slot *slots, uint32_t n_slots, const int32_t *
             opted_in = lds_bytes;
         }
     }
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, queue);
-    hipLaunchKernelGGL(k_resident<NT>, dim3(grid), dim3(NT), lds_bytes, s, slots, n_slots, slot_lats,
-                       lats, queue);
+    const uint32_t n_done = n_sub > 1u ? n_slots : 0u;
+    hipLaunchKernelGGL(k_reset_queue, dim3(n_done / 256u + 1u), dim3(256), 0, s, queue, done, n_done);
+    hipLaunchKernelGGL(k_resident<NT>, dim3(grid), dim3(NT), lds_bytes, s, slots, n_slots, n_sub, slot_lats,
+                       lats, queue, done, err);
     return hipGetLastError();
 }
 
-hipError_t launch_resident(int threads, const Slot *slots, uint32_t n_slots, const int32_t *slot_lats,
-                           PackedLattice *lats, uint32_t *queue, uint32_t lds_bytes, uint32_t grid,
-                           hipStream_t s)
+hipError_t launch_resident(int threads, const Slot *slots, uint32_t n_slots, uint32_t n_sub,
+                           const int32_t *slot_lats, PackedLattice *lats, uint32_t *queue, uint32_t *done,
+                           uint32_t *err, uint32_t lds_bytes, uint32_t grid, hipStream_t s)
 {
+    if (n_sub == 0u || (uint64_t)n_slots * n_sub > 0xFFFFFFFFull) return hipErrorInvalidValue;
     switch (threads) {
-    case 64: return launch_nt<64>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
-    case 128: return launch_nt<128>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
-    case 256: return launch_nt<256>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
-    case 512: return launch_nt<512>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
-    case 1024: return launch_nt<1024>(slots, n_slots, slot_lats, lats, queue, lds_bytes, grid, s);
+    case 64: return launch_nt<64>(slots, n_slots, n_sub, slot_lats, lats, queue, done, err, lds_bytes, grid, s);
+    case 128: return launch_nt<128>(slots, n_slots, n_sub, slot_lats, lats, queue, done, err, lds_bytes, grid, s);
+    case 256: return launch_nt<256>(slots, n_slots, n_sub, slot_lats, lats, queue, done, err, lds_bytes, grid, s);
+    case 512: return launch_nt<512>(slots, n_slots, n_sub, slot_lats, lats, queue, done, err, lds_bytes, grid, s);
+    case 1024:
+        return launch_nt<1024>(slots, n_slots, n_sub, slot_lats, lats, queue, done, err, lds_bytes, grid, s);
     default: return hipErrorInvalidValue;
     }
 }

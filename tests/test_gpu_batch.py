@@ -60,6 +60,20 @@ def test_mixed_batch_bit_exact(amd, ctx, orc, threads, chunk):
         _check_vs_oracle(orc, b, jobs)
 
 
+@pytest.mark.parametrize("threads,chunk,parts", [(256, 250, 2), (256, 13, 3), (64, 5, 7), (128, 9, 64), (512, 40, 4)])
+def test_parted_slots_bit_exact(amd, ctx, orc, threads, chunk, parts):
+    """Each slot's sweeps of a launch split into `parts` sequential parts that
+    any workgroup may take (waiting for the slot's previous part): the chains
+    are the oracle's, also with parts of no sweeps (64 parts of 9 sweeps)."""
+    jobs = _jobs(7 * parts + threads, 48, meas_max=40)
+    print("parted batch", threads, chunk, parts, flush=True)
+    with amd.Batch(ctx, jobs, threads=threads, sweeps_per_launch=chunk, sub_chunks=parts) as b:
+        assert b.options()["sub_chunks"] == parts
+        b.run_all()
+        b.sync()
+        _check_vs_oracle(orc, b, jobs)
+
+
 def test_large_lattices_bit_exact(amd, ctx, orc):
     jobs = [dict(L=256, coupling=KC, seed=11, n_therm=3, n_meas=9, n_levels=6, sweeps_per_bin=4),
             dict(L=512, coupling=0.4496804, seed=12, n_therm=2, n_meas=6, n_levels=7, sweeps_per_bin=5),
