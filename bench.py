@@ -54,18 +54,21 @@ sys.path.insert(0, ROOT)
 METRIC = "spin-updates/ns (batched vs plain) at matched y_t error; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of k_resident<threads>, keyed by (replicas, sweeps per
-# launch, threads, min_team, max_slot_lds, threads_large), from two rocprofv3
+# launch, threads, min_team, max_slot_lds, threads_large, sub_chunks), from two rocprofv3
 # --pmc passes over `bench.py --only batched` with the same configuration flags:
 # FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 KiB per dispatch, the scale factors read
 # off the same counters on a 1 GiB read + 1 GiB write copy kernel
 # (--copy-calibration).  Configurations without a counter run report null.
+# The entries were measured with every slot in one part (sub_chunks 1); the
+# default splits slots into parts, each of which reloads and writes back its
+# lattices, and has no counter run.
 # Changes to the kernel's arithmetic alone leave these unchanged: across the
 # change of the Philox key mix to one v_bitop3_b32 the default configuration
 # read 3,156.1 then 3,157.0 KiB and wrote 5,047.8 then 5,048 KiB.
 MEASURED_TRAFFIC = {
-    (4096, 100, 256, 16, 0, 0): (3156.1 * 2.0 + 5047.8) * 1024.0,
+    (4096, 100, 256, 16, 0, 0, 1): (3156.1 * 2.0 + 5047.8) * 1024.0,
     # two launch classes here (the L = 8 slots grow past 64 KiB): sums over both
-    (4096, 100, 256, 16, 65536, 0): (3343.1 * 2.0 + 5046.9) * 1024.0,
+    (4096, 100, 256, 16, 65536, 0, 1): (3343.1 * 2.0 + 5046.9) * 1024.0,
 }
 # The same kernel is bound by VALU issue, not HBM: rocprofv3 --pmc SQ_INSTS_VALU
 # and GRBM_GUI_ACTIVE over the same launches give the wave-level VALU
@@ -75,7 +78,7 @@ MEASURED_TRAFFIC = {
 # blocking and plaquette parity terms have since taken the same form and the
 # register pyramid sums fewer lanes, so the current count is at most this.
 MEASURED_VALU = {
-    (4096, 100, 256, 16, 0, 0): {"valu_wave_instr_per_launch": 5.5006e8, "gui_active_cycles_per_xcd": 32599700.0 / 8},
+    (4096, 100, 256, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 5.5006e8, "gui_active_cycles_per_xcd": 32599700.0 / 8},
 }
 
 
@@ -114,7 +117,8 @@ def run_batched_timed(mcrg_amd, ctx, jobs, sps, tuned, launches):
     n_sweeps = launches * sps
     bjobs = [dict(j, n_therm=0, n_meas=n_sweeps, sweeps_per_bin=max(1, n_sweeps)) for j in jobs]
     with mcrg_amd.Batch(ctx, bjobs, threads=tuned["threads"], sweeps_per_launch=sps, min_team=tuned["min_team"],
-                        max_slot_lds=tuned["max_slot_lds"], threads_large=tuned["threads_large"]) as b:
+                        max_slot_lds=tuned["max_slot_lds"], threads_large=tuned["threads_large"],
+                        sub_chunks=tuned["sub_chunks"]) as b:
         b.run(sps)  # first launch builds the slot table
         b.sync()
         n0, ms0 = ctx.kernel_time(3)
@@ -323,6 +327,8 @@ def main():
     ap.add_argument("--min-team", type=int, default=16, help="--only batched: lanes per lattice at least")
     ap.add_argument("--max-slot-lds", type=int, default=0, help="--only batched: LDS cap per slot (0 = default)")
     ap.add_argument("--threads-large", type=int, default=0, help="--only batched: workgroup size for L = 512")
+    ap.add_argument("--sub-chunks", type=int, default=0,
+                    help="parts per slot and launch (0 = automatic; the autotuned run uses 0)")
     ap.add_argument("--plain-L", type=int, default=16384, help="--only plain: lattice side")
     ap.add_argument("--plain-levels", type=int, default=1, help="--only plain: blockings")
     ap.add_argument("--plain-measure", type=int, default=0, help="--only plain: measure every sweep (1) or not (0)")
@@ -353,7 +359,8 @@ def main():
     jobs = workloads.batch_manifest(args.replicas)
     ctx = mcrg_amd.Context(0)
     if args.only == "batched":
-        cfg = dict(min_team=args.min_team, max_slot_lds=args.max_slot_lds, threads_large=args.threads_large)
+        cfg = dict(min_team=args.min_team, max_slot_lds=args.max_slot_lds, threads_large=args.threads_large,
+                   sub_chunks=args.sub_chunks)
         value, dt, _, _ = run_batched(mcrg_amd, ctx, jobs, args.steps, args.warmup, args.sweeps_per_step,
                                       args.threads, cfg=cfg)
         log(f"bench: batched {value:.1f} updates/ns ({dt:.3f} s timed), {cfg}")
@@ -376,11 +383,11 @@ def main():
                                           args.threads, autotune=args.warmup >= 10)
     log(f"bench: batched {value:.1f} updates/ns ({dt:.3f} s timed); configuration "
         f"threads={tuned['threads']} min_team={tuned['min_team']} max_slot_lds={tuned['max_slot_lds']} "
-        f"threads_large={tuned['threads_large']}")
+        f"threads_large={tuned['threads_large']} sub_chunks={tuned['sub_chunks']}")
 
     avg_ms, n_launch = run_batched_timed(mcrg_amd, ctx, jobs, args.sweeps_per_step, tuned, 11)
     cfg_key = (len(jobs), args.sweeps_per_step, tuned["threads"], tuned["min_team"], tuned["max_slot_lds"],
-               tuned["threads_large"])
+               tuned["threads_large"], tuned["sub_chunks"])
     bpl = bytes_per_update(True) * sites * args.sweeps_per_step
     achieved = bpl / (avg_ms * 1e-3) / 1e9
     log(f"bench: resident kernel {avg_ms:.3f} ms/launch over {n_launch} launches -> {achieved:.1f} GB/s")
@@ -426,7 +433,8 @@ def main():
         "config": {"workload": f"batched: {len(jobs)} replicas, L in {{8..128}}, 5 couplings, seeds 0..{len(jobs)-1}, "
                                f"bit-packed LDS-resident, blocking to L=4, every sweep measured",
                    "replicas": len(jobs), "sweeps_per_step": args.sweeps_per_step, "threads": tuned["threads"],
-                   "autotuned": {k: tuned[k] for k in ("threads", "min_team", "max_slot_lds", "threads_large")},
+                   "autotuned": {k: tuned[k] for k in ("threads", "min_team", "max_slot_lds", "threads_large",
+                                                       "sub_chunks")},
                    "sites": sites, "layout": "packed", "schedule": "resident", "device": dev["name"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
