@@ -197,7 +197,7 @@ typedef struct {
     int32_t autotune;          /* nonzero: the first launches each try one
                                   candidate configuration (timed with events)
                                   and the fastest is kept if it beats the
-                                  configuration given here by more than 2 %;
+                                  configuration given here by more than 4 %;
                                   results are the same under every
                                   configuration                                 */
     int32_t sub_chunks;        /* parts each workgroup slot's sweeps of one

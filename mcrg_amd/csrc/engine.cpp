@@ -566,8 +566,9 @@ hipError_t ResidentEngine::tick(uint32_t n, uint32_t &advanced, KernelTimer *tm)
         if (++tune_next_ == tune_.size()) {
             // entry 0 only warmed up; entry 1 (the configuration in use) is
             // replaced only by a candidate faster by more than one launch's
-            // timing noise (about 1 %), so near-equal candidates do not flip
-            // the choice from run to run
+            // timing noise (up to 3 % between launches of the batched
+            // workload), so near-equal candidates do not flip the choice from
+            // run to run
             size_t best = 2;
             for (size_t i = 3; i < tune_.size(); ++i)
                 if (tune_[i].rate > tune_[best].rate) best = i;

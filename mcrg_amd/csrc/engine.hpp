@@ -50,7 +50,7 @@ constexpr uint64_t kSweepPerSiteNs = 75;
 
 // The autotuner keeps the configuration in use unless a candidate's measured
 // rate exceeds it by this factor.
-constexpr double kTuneMargin = 1.02;
+constexpr double kTuneMargin = 1.04;
 
 struct EngineOptions {
     int threads = 256;           // workgroup size of the resident kernel
