@@ -35,8 +35,6 @@ __global__ __launch_bounds__(256) void k_packed_init(PackedLattice *__restrict__
     P.spins[j] = word;
 }
 
-__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
-
 // Waits until *flag >= want (agent-scope acquire), polling with s_sleep.
 // False when the 100 MHz real-time clock has advanced by more than
 // kSubChunkWaitTicks first.
@@ -74,12 +72,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
         }
         __syncthreads();
         // the item and its slot are the same in every lane: keep them scalar
-        const uint32_t idx = uni(s_idx);
+        const uint32_t idx = uni<true>(s_idx);
         __syncthreads();
         if (idx >= n_items) break;
-        const uint32_t k = uni(idx / n_slots), si = idx - k * n_slots;
+        const uint32_t k = uni<true>(idx / n_slots), si = idx - k * n_slots;
         Slot sl = slots[si];
-        const uint32_t per = uni((sl.n_sweeps + n_sub - 1u) / n_sub), start = k * per;
+        const uint32_t per = uni<true>((sl.n_sweeps + n_sub - 1u) / n_sub), start = k * per;
         sl.n_sweeps = start < sl.n_sweeps ? min(per, sl.n_sweeps - start) : 0u;
         bool ok = true;
         if (k > 0u) {
@@ -92,7 +90,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
             }
             __syncthreads();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            ok = uni(s_ok) != 0u;
+            ok = uni<true>(s_ok) != 0u;
         }
         if (ok && sl.n_sweeps > 0u) {
             Team tm;

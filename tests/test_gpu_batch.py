@@ -133,6 +133,14 @@ def test_invalid_jobs(amd, ctx, bad, msg):
     assert e.value.status == amd.ERR_INVALID_ARG and msg in str(e.value)
 
 
+@pytest.mark.parametrize("parts", [-1, 65])
+def test_invalid_sub_chunks(amd, ctx, parts):
+    job = dict(L=32, coupling=KC, seed=1, n_therm=1, n_meas=1, n_levels=2, sweeps_per_bin=1)
+    with pytest.raises(amd.MCRGError) as e:
+        amd.Batch(ctx, [job], sub_chunks=parts)
+    assert e.value.status == amd.ERR_INVALID_ARG and "sub_chunks" in str(e.value)
+
+
 @pytest.mark.parametrize("threads_large", [0, 256, 512, 1024])
 def test_launch_classes_mixed_lds(amd, ctx, orc, threads_large):
     """Slots of very different LDS size (L = 8 groups next to single L = 512
