@@ -221,14 +221,14 @@ def _serve(mcrg_amd, srv, stream, rate, duration, drain_limit, quantum):
         if time.perf_counter() - t0 > next_log:
             collect()
             log(f"bench: serving {rate}/s t={time.perf_counter() - t0:.0f}s submitted "
-                f"{srv.stats()['submitted']} done {n_done}")
+                f"{srv.stats()['submitted']} completed {srv.stats()['completed']} collected {n_done}")
             next_log += 10.0
     t_last_arrival = time.perf_counter() - t0
     while n_done < len(stream) and time.perf_counter() - t0 < duration + drain_limit:
         collect()
         time.sleep(0.005)
         if time.perf_counter() - t0 > next_log:
-            log(f"bench: serving {rate}/s draining t={time.perf_counter() - t0:.0f}s done {n_done}/{len(stream)}")
+            log(f"bench: serving {rate}/s draining t={time.perf_counter() - t0:.0f}s collected {n_done}/{len(stream)}")
             next_log += 10.0
     collect()
     t_end = time.perf_counter() - t0
