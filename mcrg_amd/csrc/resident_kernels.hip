@@ -2,11 +2,14 @@
 //
 //   k_packed_init   Philox +-1 start of bit-packed (CB) lattices
 //                   (Lattice::initialize_random_spins, src/lattice.cpp:33-41)
-//   k_resident      persistent grid; each workgroup pulls slots (G lattices of
-//                   one L) from an atomic queue and runs one chunk of sweeps and
-//                   measurements per slot with the lattices held in LDS
-//                   (src/ising.cpp:87-149 replaced by Metropolis; src/lattice.cpp:102-120,
-//                   src/mcrg.cpp:72-98 and :314-348 fused)
+//   k_resident      persistent grid; each workgroup pulls work items (one part
+//                   of one slot: G lattices of one L) from an atomic queue and
+//                   runs the part's sweeps and measurements with the lattices
+//                   held in LDS; a slot's parts run in order, possibly in
+//                   different workgroups (src/ising.cpp:87-149 replaced by
+//                   Metropolis; src/lattice.cpp:102-120, src/mcrg.cpp:72-98 and
+//                   :314-348 fused)
+//   k_reset_queue   zeroes a launch's work counter and per-slot progress words
 #include <mutex>
 
 #include "resident_core.hpp"
