@@ -458,9 +458,12 @@ hipError_t ResidentEngine::upload_slots()
 hipError_t ResidentEngine::check_parts()
 {
     if (!parted_) return hipSuccess;
+    // on the engine's stream, which the caller has just drained: no wait on
+    // other streams
     uint32_t err = 0;
     hipError_t e;
-    if ((e = hipMemcpy(&err, d_err_, sizeof(err), hipMemcpyDeviceToHost)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(&err, d_err_, sizeof(err), hipMemcpyDeviceToHost, stream_)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
     parted_ = false;
     if (err != 0u) {
         failed_ = "k_resident: a slot part waited for its predecessor beyond the limit";
