@@ -80,6 +80,27 @@ def test_stream_bit_exact_vs_oracle(amd, ctx, orc, chunk, quantum):
         assert e.value.status == amd.ERR_NOT_FOUND
 
 
+def test_more_slots_than_cus_bit_exact(amd, ctx, orc):
+    """320 L = 128 jobs are 320 single-lattice slots, more than the 256 CUs:
+    each launch splits every slot's sweeps into parts that other workgroups
+    may take; the jobs' 60..260 sweeps span several parts and launches, and
+    the results equal the oracle's."""
+    rng = np.random.default_rng(7)
+    jobs = [dict(L=128, coupling=float(rng.choice([0.4320459, KC, 0.4688157])), seed=5000 + i,
+                 n_therm=int(rng.integers(0, 60)), n_meas=int(rng.integers(60, 200)), n_levels=5,
+                 sweeps_per_bin=int(rng.integers(10, 70)))
+            for i in range(320)]
+    print("server parts", len(jobs), flush=True)
+    with amd.Server(ctx) as srv:
+        ids = [srv.submit(j) for j in jobs]
+        for _ in range(1000):
+            if srv.step() == 0 and srv.stats()["queued"] == 0:
+                break
+        assert srv.stats()["completed"] == len(jobs)
+        for jid, job in list(zip(ids, jobs))[::8]:
+            assert np.array_equal(srv.records(jid), _expect(orc, job)), f"records differ for {job}"
+
+
 def test_small_budget_queues_and_reuses_memory(amd, ctx, orc):
     """A budget of three L=64 lattices: admissions wait for retirements, and the
     reused device memory starts clean for every job."""
