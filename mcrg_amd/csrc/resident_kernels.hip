@@ -40,8 +40,10 @@ __global__ __launch_bounds__(256) void k_packed_init(PackedLattice *__restrict__
 
 // Waits until *flag >= want (agent-scope acquire), polling with s_sleep.
 // False when the 100 MHz real-time clock has advanced by more than
-// kSubChunkWaitTicks first.
-__device__ bool wait_flag(const uint32_t *flag, uint32_t want)
+// kSubChunkWaitTicks first.  Not inlined: a leaf call with no stack, which
+// keeps k_resident at 96 VGPRs without spilling (inlined, the loop's state
+// cost three spilled registers in the whole kernel).
+__device__ __noinline__ bool wait_flag(const uint32_t *flag, uint32_t want)
 {
     const uint64_t t0 = (uint64_t)wall_clock64();
     while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
