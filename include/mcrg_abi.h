@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define MCRG_ABI_VERSION 2
+#define MCRG_ABI_VERSION 3
 
 typedef int32_t mcrg_status;
 enum {
@@ -77,6 +77,15 @@ mcrg_status mcrg_bond_threshold(double coupling, uint32_t *thr);
  * src/definitions.cpp:9-19) and SLb_SL_avg_loc of approx_critical_point
  * (src/mcrg.cpp:262-263), kept exactly in int64 per bin. */
 int32_t mcrg_record_size(int32_t n_levels);
+
+/* Largest sweeps_per_bin for which one bin's int64 entries cannot overflow on a
+ * side-L lattice: floor((2^63 - 1) / (2 L^2)^2), the bound of a level-0 self
+ * product (2L^2 single-counted bonds, all aligned) summed over the bin's
+ * samples (0 for an L out of range).  Chains, batch jobs and server jobs with
+ * more sweeps per bin are rejected with MCRG_ERR_INVALID_ARG.  Sums over bins
+ * are formed in 128-bit integers by the solve functions below.  No reference
+ * counterpart: the reference keeps its sums in FP64 (src/mcrg.cpp:59-66). */
+int64_t mcrg_max_sweeps_per_bin(int32_t L);
 
 /* A context owns one device.  Replaces the MPI_Init / MPI_COMM_WORLD setup of
  * src/main.cpp:6 (one process per device; no MPI). */
@@ -366,6 +375,11 @@ mcrg_status mcrg_tau_int(const int64_t *records, int32_t n_bins, int32_t n_level
  * its sign convention (K = -J), i.e. J_new = J - dK. */
 mcrg_status mcrg_critical_point_step(const int64_t *rec_large, int32_t nl_large, const int64_t *rec_small,
                                      int32_t nl_small, int32_t level, double *dK);
+/* The same step from n_bins per-bin records of each chain (summed in 128-bit
+ * integers, so totals beyond 2^63 stay exact). */
+mcrg_status mcrg_critical_point_step_bins(const int64_t *rec_large, int32_t n_bins_large, int32_t nl_large,
+                                          const int64_t *rec_small, int32_t n_bins_small, int32_t nl_small,
+                                          int32_t level, double *dK);
 
 /* ------------------------------------------------------------------ timing */
 /* Kernel-duration statistics measured with HIP events on the stream that

@@ -145,6 +145,9 @@ _SIGS = {
     "mcrg_rg_exponent": (_c.c_int32, [_p, _c.c_int32, _c.c_int32, _c.c_int32, _c.c_int32, _c.POINTER(RGResult)]),
     "mcrg_critical_point_step": (_c.c_int32, [_p, _c.c_int32, _p, _c.c_int32, _c.c_int32,
                                               _c.POINTER(_c.c_double)]),
+    "mcrg_critical_point_step_bins": (_c.c_int32, [_p, _c.c_int32, _c.c_int32, _p, _c.c_int32, _c.c_int32,
+                                                   _c.c_int32, _c.POINTER(_c.c_double)]),
+    "mcrg_max_sweeps_per_bin": (_c.c_int64, [_c.c_int32]),
     "mcrg_tau_int": (_c.c_int32, [_p, _c.c_int32, _c.c_int32, _c.c_int32, _c.c_int32, _c.POINTER(TauResult)]),
 }
 for _name, (_res, _args) in _SIGS.items():
@@ -228,11 +231,20 @@ def tau_int(records, n_levels, level=0, op=OP_NN):
 
 
 def critical_point_step(rec_large, nl_large, rec_small, nl_small, level):
-    a = np.ascontiguousarray(rec_large, dtype=np.int64).reshape(-1)
-    b = np.ascontiguousarray(rec_small, dtype=np.int64).reshape(-1)
+    """The reference's locator step dK from one record of each chain; with
+    several bins per array (shape (bins, record_size)) the bins are summed in
+    128-bit integers."""
+    a = np.ascontiguousarray(rec_large, dtype=np.int64).reshape(-1, record_size(nl_large))
+    b = np.ascontiguousarray(rec_small, dtype=np.int64).reshape(-1, record_size(nl_small))
     d = ctypes.c_double()
-    check(_lib.mcrg_critical_point_step(_ptr(a), nl_large, _ptr(b), nl_small, level, ctypes.byref(d)))
+    check(_lib.mcrg_critical_point_step_bins(_ptr(a), a.shape[0], nl_large, _ptr(b), b.shape[0], nl_small, level,
+                                             ctypes.byref(d)))
     return d.value
+
+
+def max_sweeps_per_bin(L):
+    """Most sweeps per record bin whose int64 sums cannot overflow at side L."""
+    return _lib.mcrg_max_sweeps_per_bin(L)
 
 
 class Context:
@@ -275,7 +287,9 @@ class Chain:
     """One Markov chain on the device (mcrg_chain)."""
 
     def __init__(self, ctx, L, coupling, seed, n_levels=0, layout=LAYOUT_INT8,
-                 sweeps_per_bin=1000, max_bins=1, sampler=SAMPLER_METROPOLIS):
+                 sweeps_per_bin=None, max_bins=1, sampler=SAMPLER_METROPOLIS):
+        if sweeps_per_bin is None:  # 1000, or fewer where one bin's int64 sums could overflow
+            sweeps_per_bin = max(1, min(1000, max_sweeps_per_bin(L)))
         d = ChainDesc(L, float(coupling), seed, n_levels, layout, sweeps_per_bin, max_bins, sampler)
         h = ctypes.c_void_p()
         check(_lib.mcrg_chain_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)))

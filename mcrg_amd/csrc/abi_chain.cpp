@@ -205,6 +205,7 @@ mcrg_status mcrg_chain_create(mcrg_ctx ctx, const mcrg_chain_desc *desc, mcrg_ch
     MCRG_REQUIRE(d.n_levels >= 0 && d.n_levels <= ilog2(d.L) - 1 && d.n_levels <= (int)kMaxLevels,
                  "n_levels must be in [0, log2(L) - 1]");
     MCRG_REQUIRE(d.sweeps_per_bin >= 1 && d.max_bins >= 1, "sweeps_per_bin and max_bins must be >= 1");
+    MCRG_REQUIRE(bin_fits(d.L, d.sweeps_per_bin), MCRG_BIN_LIMIT_MSG);
     MCRG_REQUIRE(d.layout == MCRG_LAYOUT_INT8, "only MCRG_LAYOUT_INT8 chains run on this schedule");
     MCRG_REQUIRE(d.sampler == MCRG_SAMPLER_METROPOLIS || d.sampler == MCRG_SAMPLER_SWENDSEN_WANG,
                  "sampler must be MCRG_SAMPLER_METROPOLIS or MCRG_SAMPLER_SWENDSEN_WANG");

@@ -124,6 +124,7 @@ std::string ResidentEngine::check(const JobSpec &j)
     if (!(j.coupling >= 0.0) || !std::isfinite(j.coupling)) return "coupling J = |K| must be >= 0";
     if (j.n_levels < 0 || j.n_levels > ilog2(j.L) - 1) return "n_levels must be in [0, log2(L) - 1]";
     if (j.sweeps_per_bin < 1) return "sweeps_per_bin must be >= 1";
+    if (!bin_fits(j.L, j.sweeps_per_bin)) return MCRG_BIN_LIMIT_MSG;
     if ((uint64_t)j.n_therm + (uint64_t)j.n_meas > 0xFFFFFFFFull) return "too many sweeps";
     if (j.sampler != MCRG_SAMPLER_METROPOLIS) return "the LDS-resident schedule runs the Metropolis sampler only";
     return std::string();
@@ -188,6 +189,12 @@ uint32_t ResidentEngine::lds_share(int L, int nl, int NT) const
 {
     const int G = group_size(L, nl, NT);
     return (lds_bytes(L, G, nl, NT) + (uint32_t)G - 1u) / (uint32_t)G;
+}
+
+uint64_t ResidentEngine::slot_time_ns(int L, int G, uint32_t n_sweeps) const
+{
+    const uint64_t sites_per_lane = (uint64_t)G * (uint64_t)L * (uint64_t)L / (uint64_t)threads_for(L);
+    return (uint64_t)n_sweeps * (kSweepFixedNs + kSweepPerSiteNs * sites_per_lane);
 }
 
 uint32_t ResidentEngine::slot_sweeps(int L, int G, uint32_t chunk) const
@@ -451,6 +458,17 @@ hipError_t ResidentEngine::upload_slots()
             lc.n_sub = std::max(1u, std::min(4u, min_sweeps / 16u));
         else
             lc.n_sub = 1;
+        // A part waits at most for the parts of its slot queued before it,
+        // i.e. for less than the slot's whole time in this launch.  Parting is
+        // only a balance aid (results do not depend on it), so a class whose
+        // longest slot is modelled above kPartedSlotCapNs runs unparted and no
+        // wait can approach the kernel's limit.
+        if (lc.n_sub > 1u) {
+            uint64_t worst = 0;
+            for (uint32_t i = lc.first; i < lc.first + lc.count; ++i)
+                worst = std::max(worst, slot_time_ns(slots_[i].L, slots_[i].G, slots_[i].n_sweeps));
+            if (worst > kPartedSlotCapNs) lc.n_sub = 1;
+        }
     }
     return hipSuccess;
 }
@@ -466,6 +484,11 @@ hipError_t ResidentEngine::check_parts()
     if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
     parted_ = false;
     if (err != 0u) {
+        // reported once: the word is cleared (a plain copy on the drained
+        // stream, outside any graph) so that later launches are judged alone
+        static const uint32_t zero = 0u;
+        if ((e = hipMemcpyAsync(d_err_, &zero, sizeof(zero), hipMemcpyHostToDevice, stream_)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
         failed_ = "k_resident: a slot part waited for its predecessor beyond the limit";
         return hipErrorLaunchFailure;
     }

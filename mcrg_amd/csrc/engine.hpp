@@ -52,6 +52,10 @@ constexpr uint64_t kSweepPerSiteNs = 75;
 // rate exceeds it by this factor.
 constexpr double kTuneMargin = 1.04;
 
+// Longest modelled slot time for which a launch splits slots into parts: a
+// tenth of the part wait's 20 s limit.
+constexpr uint64_t kPartedSlotCapNs = 2000000000ull;
+
 struct EngineOptions {
     int threads = 256;           // workgroup size of the resident kernel
     int sweeps_per_launch = 250;  // chunk length (the most sweeps any lattice runs per launch)
@@ -164,6 +168,8 @@ class ResidentEngine {
     hipEvent_t tune_a_ = nullptr, tune_b_ = nullptr;
     void rebuild_slots(uint32_t chunk);
     uint32_t slot_sweeps(int L, int G, uint32_t chunk) const;
+    // Modelled time of n_sweeps measured sweeps of a slot (the cost model above).
+    uint64_t slot_time_ns(int L, int G, uint32_t n_sweeps) const;
     hipError_t launch_classes(KernelTimer *tm);
 
     // Slots whose LDS allows the same number of workgroups per CU form one

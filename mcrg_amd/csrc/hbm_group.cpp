@@ -137,6 +137,7 @@ std::string HbmGroup::check(const JobSpec &j)
     if (j.n_levels < 0 || j.n_levels > ilog2(j.L) - 1 || j.n_levels > (int)kMaxLevels)
         return "n_levels must be in [0, log2(L) - 1]";
     if (j.sweeps_per_bin < 1) return "sweeps_per_bin must be >= 1";
+    if (!bin_fits(j.L, j.sweeps_per_bin)) return MCRG_BIN_LIMIT_MSG;
     if ((uint64_t)j.n_therm + (uint64_t)j.n_meas > 0xFFFFFFFFull) return "too many sweeps";
     if (j.sampler != MCRG_SAMPLER_METROPOLIS && j.sampler != MCRG_SAMPLER_SWENDSEN_WANG)
         return "sampler must be MCRG_SAMPLER_METROPOLIS or MCRG_SAMPLER_SWENDSEN_WANG";

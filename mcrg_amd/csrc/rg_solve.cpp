@@ -17,15 +17,29 @@ namespace {
 
 using i128 = __int128;
 
+// Record totals in 128-bit integers: one bin of int64 entries is exact by
+// construction (jobs are admitted only while sweeps_per_bin (2L^2)^2 < 2^63),
+// but sums over many bins or replicas can pass 2^63 (NN self sums near K_c are
+// about 2 L^4 per sample: at L = 2048 and 1e6 samples, 3.5e19).
 struct Sums {
-    const int64_t *r;
+    const i128 *r;
     uint32_t nl;
-    int64_t S(uint32_t l, uint32_t a) const { return r[off_S(l, a)]; }
-    int64_t self(uint32_t l, uint32_t a, uint32_t b) const { return r[off_self(nl, l, a, b)]; }
-    int64_t cross(uint32_t l, uint32_t a, uint32_t b) const { return r[off_cross(nl, l, a, b)]; }
-    int64_t nn0(uint32_t l) const { return r[off_nn0(nl, l)]; }
-    int64_t n() const { return r[off_M(nl) + 3u]; }
+    i128 S(uint32_t l, uint32_t a) const { return r[off_S(l, a)]; }
+    i128 self(uint32_t l, uint32_t a, uint32_t b) const { return r[off_self(nl, l, a, b)]; }
+    i128 cross(uint32_t l, uint32_t a, uint32_t b) const { return r[off_cross(nl, l, a, b)]; }
+    i128 nn0(uint32_t l) const { return r[off_nn0(nl, l)]; }
+    i128 n() const { return r[off_M(nl) + 3u]; }
 };
+
+// Sum of n_bins records of record_size(nl) entries each.
+std::vector<i128> total_of(const int64_t *records, int64_t n_bins, uint32_t nl)
+{
+    const uint32_t rs = record_size(nl);
+    std::vector<i128> t(rs, 0);
+    for (int64_t b = 0; b < n_bins; ++b)
+        for (uint32_t k = 0; k < rs; ++k) t[k] += records[(size_t)b * rs + k];
+    return t;
+}
 
 // T = A^-1 B for A, B of size n <= 3 (row-major), by the adjugate of A.
 bool solve_small(int n, const double *A, const double *B, double *T)
@@ -110,7 +124,20 @@ double leading_eigenvalue(int n, const double *T)
     return r > other ? r : other;
 }
 
-bool rg_matrix(const int64_t *rec, uint32_t nl, uint32_t level, int n_ops, double *T)
+// n sum(xy) - sum(x) sum(y) as a double.  While every total is below 2^63 in
+// magnitude (at L = 16384 that is 30 samples per bin times 2^31 bins) the
+// products and their difference are exact 128-bit integers; beyond, they are
+// formed in long double, whose 64-bit mantissa keeps the difference's relative
+// error near 2^-64 of the terms, far below any statistical error.
+double connected(i128 n, i128 sxy, i128 sx, i128 sy)
+{
+    const i128 lim = (i128)1 << 63;
+    auto small = [&](i128 v) { return v < lim && v > -lim; };
+    if (small(n) && small(sxy) && small(sx) && small(sy)) return (double)(n * sxy - sx * sy);
+    return (double)((long double)n * (long double)sxy - (long double)sx * (long double)sy);
+}
+
+bool rg_matrix(const i128 *rec, uint32_t nl, uint32_t level, int n_ops, double *T)
 {
     const Sums s{rec, nl};
     const i128 n = s.n();
@@ -120,13 +147,13 @@ bool rg_matrix(const int64_t *rec, uint32_t nl, uint32_t level, int n_ops, doubl
         for (int b = 0; b < n_ops; ++b) {
             const i128 hi_a = s.S(level + 1, (uint32_t)a), hi_b = s.S(level + 1, (uint32_t)b);
             const i128 lo_b = s.S(level, (uint32_t)b);
-            A[a * n_ops + b] = (double)(n * (i128)s.self(level + 1, (uint32_t)a, (uint32_t)b) - hi_a * hi_b);
-            B[a * n_ops + b] = (double)(n * (i128)s.cross(level + 1, (uint32_t)a, (uint32_t)b) - hi_a * lo_b);
+            A[a * n_ops + b] = connected(n, s.self(level + 1, (uint32_t)a, (uint32_t)b), hi_a, hi_b);
+            B[a * n_ops + b] = connected(n, s.cross(level + 1, (uint32_t)a, (uint32_t)b), hi_a, lo_b);
         }
     return solve_small(n_ops, A, B, T);
 }
 
-bool exponent(const int64_t *rec, uint32_t nl, uint32_t level, int n_ops, double &lambda)
+bool exponent(const i128 *rec, uint32_t nl, uint32_t level, int n_ops, double &lambda)
 {
     double T[9];
     if (!rg_matrix(rec, nl, level, n_ops, T)) return false;
@@ -138,17 +165,17 @@ bool exponent(const int64_t *rec, uint32_t nl, uint32_t level, int n_ops, double
 // level n from the totals of an L chain (nl_l levels) and an L/2 chain.  The
 // reference sums double-counted NN (2 x single count) in the numerator and in
 // both covariances, so the step is formed from those values.
-bool kc_step(const int64_t *rl, uint32_t nll, const int64_t *rs, uint32_t nls, uint32_t n, double &dK)
+bool kc_step(const i128 *rl, uint32_t nll, const i128 *rs, uint32_t nls, uint32_t n, double &dK)
 {
     const Sums L{rl, nll}, S{rs, nls};
     const i128 nL = L.n(), nS = S.n();
     if (nL < 2 || nS < 2) return false;
-    const i128 sl = 2 * (i128)L.S(0, 0), slb = 2 * (i128)L.S(n + 1, 0), slb_sl = 4 * (i128)L.nn0(n + 1);
-    const i128 ss = 2 * (i128)S.S(0, 0), ssb = 2 * (i128)S.S(n, 0);
-    const i128 ssb_ss = 4 * (i128)(n == 0 ? S.self(0, 0, 0) : S.nn0(n));
+    const i128 sl = 2 * L.S(0, 0), slb = 2 * L.S(n + 1, 0), slb_sl = 4 * L.nn0(n + 1);
+    const i128 ss = 2 * S.S(0, 0), ssb = 2 * S.S(n, 0);
+    const i128 ssb_ss = 4 * (n == 0 ? S.self(0, 0, 0) : S.nn0(n));
     const double mean_lb = (double)slb / (double)nL, mean_sb = (double)ssb / (double)nS;
-    const double cov_l = (double)(nL * slb_sl - slb * sl) / ((double)nL * (double)nL);
-    const double cov_s = (double)(nS * ssb_ss - ssb * ss) / ((double)nS * (double)nS);
+    const double cov_l = connected(nL, slb_sl, slb, sl) / ((double)nL * (double)nL);
+    const double cov_s = connected(nS, ssb_ss, ssb, ss) / ((double)nS * (double)nS);
     if (cov_l - cov_s == 0.0) return false;
     dK = (mean_lb - mean_sb) / (cov_l - cov_s);
     return true;
@@ -167,7 +194,8 @@ mcrg_status mcrg_rg_matrix(const int64_t *record, int32_t n_levels, int32_t leve
     MCRG_REQUIRE(n_levels >= 1 && n_levels <= (int32_t)kMaxLevels, "n_levels out of range");
     MCRG_REQUIRE(level >= 0 && level < n_levels, "level must be in [0, n_levels)");
     MCRG_REQUIRE(n_ops >= 1 && n_ops <= 3, "n_ops must be 1, 2 or 3");
-    if (!rg_matrix(record, (uint32_t)n_levels, (uint32_t)level, n_ops, T)) {
+    const std::vector<i128> tot = total_of(record, 1, (uint32_t)n_levels);
+    if (!rg_matrix(tot.data(), (uint32_t)n_levels, (uint32_t)level, n_ops, T)) {
         set_error("RG matrix undefined (fewer than 2 samples or singular correlations)");
         return MCRG_ERR_INVALID_ARG;
     }
@@ -189,9 +217,8 @@ mcrg_status mcrg_rg_exponent(const int64_t *records, int32_t n_bins, int32_t n_l
     MCRG_REQUIRE(level >= 0 && level < n_levels, "level must be in [0, n_levels)");
     MCRG_REQUIRE(n_ops >= 1 && n_ops <= 3, "n_ops must be 1, 2 or 3");
     const uint32_t nl = (uint32_t)n_levels, rs = record_size(nl);
-    std::vector<int64_t> tot(rs, 0), loo(rs);
-    for (int32_t b = 0; b < n_bins; ++b)
-        for (uint32_t k = 0; k < rs; ++k) tot[k] += records[(size_t)b * rs + k];
+    const std::vector<i128> tot = total_of(records, n_bins, nl);
+    std::vector<i128> loo(rs);
     double lam = 0.0;
     if (!exponent(tot.data(), nl, (uint32_t)level, n_ops, lam)) {
         set_error("RG exponent undefined (too few samples, singular correlations or lambda <= 0)");
@@ -257,7 +284,7 @@ mcrg_status mcrg_tau_int(const int64_t *records, int32_t n_bins, int32_t n_level
     }
     const i128 n = (i128)c * nb;
     // per-sample variance from exact sums: (n q - s^2) / n^2
-    const double var = (double)(n * q - s * s) / ((double)n * (double)n);
+    const double var = connected(n, q, s, s) / ((double)n * (double)n);
     MCRG_REQUIRE(var > 0.0, "the observable does not fluctuate");
     auto tau_of = [&](const std::vector<double> &mb, double B) {
         double mean = 0.0;
@@ -287,18 +314,38 @@ mcrg_status mcrg_tau_int(const int64_t *records, int32_t n_bins, int32_t n_level
     return MCRG_OK;
 }
 
-mcrg_status mcrg_critical_point_step(const int64_t *rec_large, int32_t nl_large, const int64_t *rec_small,
-                                     int32_t nl_small, int32_t level, double *dK)
+mcrg_status mcrg_critical_point_step_bins(const int64_t *rec_large, int32_t n_bins_large, int32_t nl_large,
+                                          const int64_t *rec_small, int32_t n_bins_small, int32_t nl_small,
+                                          int32_t level, double *dK)
 {
     MCRG_REQUIRE(rec_large != nullptr && rec_small != nullptr && dK != nullptr, "null argument");
+    MCRG_REQUIRE(n_bins_large >= 1 && n_bins_small >= 1, "n_bins must be >= 1");
     MCRG_REQUIRE(level >= 0 && level + 1 <= nl_large && level <= nl_small,
                  "level needs level+1 blockings of the large chain and level of the small one");
     MCRG_REQUIRE(nl_large <= (int32_t)kMaxLevels && nl_small <= (int32_t)kMaxLevels, "n_levels out of range");
-    if (!kc_step(rec_large, (uint32_t)nl_large, rec_small, (uint32_t)nl_small, (uint32_t)level, *dK)) {
+    const std::vector<i128> tl = total_of(rec_large, n_bins_large, (uint32_t)nl_large);
+    const std::vector<i128> ts = total_of(rec_small, n_bins_small, (uint32_t)nl_small);
+    if (!kc_step(tl.data(), (uint32_t)nl_large, ts.data(), (uint32_t)nl_small, (uint32_t)level, *dK)) {
         set_error("critical-point step undefined (too few samples or equal covariances)");
         return MCRG_ERR_INVALID_ARG;
     }
     return MCRG_OK;
+}
+
+mcrg_status mcrg_critical_point_step(const int64_t *rec_large, int32_t nl_large, const int64_t *rec_small,
+                                     int32_t nl_small, int32_t level, double *dK)
+{
+    return mcrg_critical_point_step_bins(rec_large, 1, nl_large, rec_small, 1, nl_small, level, dK);
+}
+
+int64_t mcrg_max_sweeps_per_bin(int32_t L)
+{
+    if (L < 2 || L > 16384) return 0;
+    // the largest per-sample entry is a level-0 self product, at most (2L^2)^2
+    const i128 L2 = (i128)L * L, worst = 4 * L2 * L2;
+    const i128 lim = ((i128)1 << 63) - 1;
+    const i128 m = lim / worst;
+    return m > (i128)0x7FFFFFFF ? (int64_t)0x7FFFFFFF : (int64_t)m;
 }
 
 }  // extern "C"

@@ -33,6 +33,9 @@ void set_error(const std::string &msg);
     } while (0)
 
 inline bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+// One bin's int64 entries cannot overflow: sweeps_per_bin (2L^2)^2 < 2^63.
+inline bool bin_fits(int L, int sweeps_per_bin) { return (int64_t)sweeps_per_bin <= mcrg_max_sweeps_per_bin(L); }
+#define MCRG_BIN_LIMIT_MSG "sweeps_per_bin (2 L^2)^2 must stay below 2^63 (see mcrg_max_sweeps_per_bin)"
 inline int ilog2(int64_t v)
 {
     int r = 0;

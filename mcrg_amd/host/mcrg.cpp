@@ -329,6 +329,13 @@ void MonteCarloRenormalizationGroup::set_sampler(int sampler)
     sampler_ = sampler;
 }
 
+void MonteCarloRenormalizationGroup::set_layout(int layout)
+{
+    if (layout != -1 && layout != MCRG_LAYOUT_INT8 && layout != MCRG_LAYOUT_PACKED)
+        throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "layout must be -1 (auto), MCRG_LAYOUT_INT8 or MCRG_LAYOUT_PACKED");
+    layout_ = layout;
+}
+
 std::vector<int64_t> MonteCarloRenormalizationGroup::run_chains(int N, double K, int n_levels, int n_samples_eq,
                                                                 int n_samples, int &n_bins)
 {
@@ -340,10 +347,50 @@ std::vector<int64_t> MonteCarloRenormalizationGroup::run_chains(int N, double K,
         n_loc[(size_t)r] = std::max(0, mcrg_cpp::split_samples(r, R, n_samples));
         max_loc = std::max(max_loc, n_loc[(size_t)r]);
     }
-    const int spb = std::max(1, (max_loc + bins_ - 1) / bins_);
+    // Bin b of the result is the sum of bin b of every replica, kept in int64:
+    // each bin holds at most R * spb samples, so spb is capped at
+    // mcrg_max_sweeps_per_bin(N) / R (more bins than asked for when needed).
+    const int64_t spb_cap = mcrg_max_sweeps_per_bin(N) / R;
+    if (spb_cap < 1)
+        throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG,
+                              "too many replicas at this N: one summed sample per replica would overflow int64");
+    const int spb = (int)std::min<int64_t>(std::max(1, (max_loc + bins_ - 1) / bins_), spb_cap);
     n_bins = std::max(1, (max_loc + spb - 1) / spb);
     std::vector<int64_t> total((size_t)n_bins * rs, 0);
     const double J = -K;
+    if (layout_ == MCRG_LAYOUT_PACKED && (N < 8 || N > 512 || sampler_ != MCRG_SAMPLER_METROPOLIS))
+        throw mcrg_cpp::Error(MCRG_ERR_INVALID_ARG, "the bit-packed layout runs Metropolis at 8 <= N <= 512 only");
+    if (layout_ == MCRG_LAYOUT_INT8) {
+        // one int8 chain per replica, each on its own stream
+        std::vector<mcrg_chain> chs((size_t)R, nullptr);
+        try {
+            for (int r = 0; r < R; ++r) {
+                mcrg_chain_desc d{};
+                d.L = N;
+                d.coupling = J;
+                d.seed = mcrg_cpp::next_seed();
+                d.n_levels = n_levels;
+                d.layout = MCRG_LAYOUT_INT8;
+                d.sweeps_per_bin = spb;
+                d.max_bins = std::max(1, (n_loc[(size_t)r] + spb - 1) / spb);
+                d.sampler = sampler_;
+                check(mcrg_chain_create(mcrg_cpp::context(), &d, &chs[(size_t)r]));
+                check(mcrg_chain_sweep(chs[(size_t)r], n_samples_eq, 0));
+                if (n_loc[(size_t)r] > 0) check(mcrg_chain_sweep(chs[(size_t)r], n_loc[(size_t)r], 1));
+            }
+            for (int r = 0; r < R; ++r) {
+                std::vector<int64_t> rec((size_t)std::max(1, (n_loc[(size_t)r] + spb - 1) / spb) * rs);
+                check(mcrg_chain_get_records(chs[(size_t)r], rec.data(), (int64_t)rec.size()));
+                for (size_t k = 0; k < rec.size(); ++k) total[k] += rec[k];
+            }
+        } catch (...) {
+            for (mcrg_chain c : chs)
+                if (c) mcrg_chain_destroy(c);
+            throw;
+        }
+        for (mcrg_chain c : chs) mcrg_chain_destroy(c);
+        return total;
+    }
     {
         // replicas advance together in one batch (LDS-resident for 8 <= N <= 512,
         // one HBM-schedule group otherwise)
@@ -591,7 +638,7 @@ double MonteCarloRenormalizationGroup::approx_critical_point(int n_samples_eq, i
     double Kc = K;
     for (int n = 0; n < n_t; ++n) {
         double dK = 0.0;
-        check(mcrg_critical_point_step(rl.data(), n_t, rs.data(), n_t - 1, n, &dK));
+        check(mcrg_critical_point_step_bins(rl.data(), nb_l, n_t, rs.data(), nb_s, n_t - 1, n, &dK));
         Kc = K + dK;
         last_kc_[(size_t)n] = Kc;
         if (!quiet_) std::printf("n = %i: Kc = %lf\n", n, Kc);

@@ -151,6 +151,12 @@ class MonteCarloRenormalizationGroup {
     void set_output_dir(const std::string &dir);
     void set_quiet(bool quiet);
     void set_sampler(int sampler);    // MCRG_SAMPLER_* for calc_critical_exponent and the locator
+    // Storage of the replicas' lattices: -1 = automatic (bit-packed in LDS for
+    // Metropolis at 8 <= N <= 512, int8 in device memory otherwise);
+    // MCRG_LAYOUT_INT8 = one int8 chain per replica (mcrg_chain_*);
+    // MCRG_LAYOUT_PACKED = the bit-packed schedule, an error where it cannot
+    // run.  Results are bit-identical under every layout.
+    void set_layout(int layout);
     // exponents of the last calc_critical_exponent, one per level
     const std::vector<mcrg_rg_result> &results() const { return results_; }
     // per-level estimates of the last approx_critical_point
@@ -195,6 +201,7 @@ class MonteCarloRenormalizationGroup {
     int n_processes_ = 1;
     int n_ops_ = 2;
     int sampler_ = MCRG_SAMPLER_METROPOLIS;
+    int layout_ = -1;
     int bins_ = 50;
     int levels_ = -1;
     int iter_ = 0;

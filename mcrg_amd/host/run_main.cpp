@@ -19,7 +19,7 @@ static void usage()
         "usage: mcrg_run [--N n] [--eq n] [--samples n] [--K k ...] [--replicas r] [--ops 2|3]\n"
         "                [--bins b] [--levels l] [--seed s] [--device d] [--outdir dir] [--quiet]\n"
         "                [--locate iterations --L l --K0 k]\n"
-        "                [--sampler metropolis|sw]\n"
+        "                [--sampler metropolis|sw] [--layout auto|int8|packed]\n"
         "                [--bisect rounds --n-k n --J-lo j --J-hi j --n-large n --L-large l\n"
         "                 --n-small n --L-small l --sampler-large sw|metropolis --eq-small n]\n"
         "                (--bisect prints one JSON line)\n"
@@ -33,6 +33,7 @@ int main(int argc, char **argv)
     int locate = 0, Lloc = 16;
     int bisect = 0, n_k = 16, n_large = 64, L_large = 1024, n_small = 1024, L_small = 128;
     double J_lo = 0.43, J_hi = 0.45;
+    int layout = -1;
     int sampler = MCRG_SAMPLER_METROPOLIS, sampler_large = MCRG_SAMPLER_SWENDSEN_WANG, n_eq_small = -1;
     double K0 = -0.44;
     unsigned long long seed = 1;
@@ -71,6 +72,16 @@ int main(int argc, char **argv)
         else if (a == "--sampler-large")
             sampler_large = std::string(next()) == "sw" ? MCRG_SAMPLER_SWENDSEN_WANG : MCRG_SAMPLER_METROPOLIS;
         else if (a == "--eq-small") n_eq_small = std::atoi(next());
+        else if (a == "--layout") {
+            const std::string v = next();
+            if (v == "int8") layout = MCRG_LAYOUT_INT8;
+            else if (v == "packed") layout = MCRG_LAYOUT_PACKED;
+            else if (v == "auto") layout = -1;
+            else {
+                usage();
+                return 2;
+            }
+        }
         else if (a == "--J-lo") J_lo = std::atof(next());
         else if (a == "--J-hi") J_hi = std::atof(next());
         else if (a == "--n-large") n_large = std::atoi(next());
@@ -97,6 +108,7 @@ int main(int argc, char **argv)
         MCRG.set_output_dir(outdir);
         MCRG.set_quiet(quiet);
         MCRG.set_sampler(sampler);
+        MCRG.set_layout(layout);
         if (bisect > 0) {
             const auto r = MCRG.bisect_critical_point(bisect, n_k, J_lo, J_hi, n_large, L_large, n_small, L_small,
                                                       n_eq, n, sampler_large, n_eq_small);

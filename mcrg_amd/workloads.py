@@ -65,21 +65,3 @@ def serving_stream(rate, duration=60.0, n_therm=1000, stream_seed=2024, sweeps_p
                             n_levels=levels_to(L), sweeps_per_bin=sweeps_per_bin)))
         j += 1
     return out
-
-
-def bisection_round(bracket, n_k=16, n_large=64, n_small=1024, L_large=1024, L_small=128,
-                    n_therm=2000, n_meas=10000, round_index=0, sweeps_per_bin=500):
-    """One round of the critical-coupling bisection: n_k couplings spanning the
-    bracket; per coupling n_large/n_k lattices of L_large (int8, HBM) and
-    n_small/n_k of L_small (bit-packed, LDS)."""
-    lo, hi = bracket
-    ks = [lo + (hi - lo) * (q + 0.5) / n_k for q in range(n_k)]
-    large, small = [], []
-    for q, K in enumerate(ks):
-        for r in range(n_large // n_k):
-            large.append(dict(L=L_large, coupling=K, seed=(round_index << 40) | (q << 20) | r,
-                              n_therm=n_therm, n_meas=n_meas, n_levels=4, sweeps_per_bin=sweeps_per_bin))
-        for r in range(n_small // n_k):
-            small.append(dict(L=L_small, coupling=K, seed=(round_index << 40) | (1 << 39) | (q << 20) | r,
-                              n_therm=n_therm, n_meas=n_meas, n_levels=1, sweeps_per_bin=sweeps_per_bin))
-    return ks, large, small

@@ -41,7 +41,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version():
     import mcrg_amd
-    assert mcrg_amd.abi_version() == 2
+    assert mcrg_amd.abi_version() == 3
 
 
 @pytest.mark.parametrize("J", [0.0, 0.1, 0.42, 0.4406868, 0.4688157, 0.489652, 1.0, 9.0])

@@ -124,7 +124,10 @@ def test_incremental_runs(amd, ctx):
 @pytest.mark.parametrize("bad,msg", [(dict(L=12), "power of two"), (dict(L=32768), "[2, 16384]"),
                                      (dict(L=4), "n_levels"),
                                      (dict(n_levels=6), "n_levels"), (dict(coupling=-0.1), "coupling"),
-                                     (dict(sweeps_per_bin=0), "sweeps_per_bin")])
+                                     (dict(sweeps_per_bin=0), "sweeps_per_bin"),
+                                     # one bin of 2^17 samples at L = 2048 could pass 2^63 (int64 records)
+                                     (dict(L=2048, sweeps_per_bin=1 << 17), "2^63"),
+                                     (dict(L=256, sweeps_per_bin=(1 << 31) - 1, n_meas=1), "2^63")])
 def test_invalid_jobs(amd, ctx, bad, msg):
     job = dict(L=32, coupling=KC, seed=1, n_therm=1, n_meas=1, n_levels=2, sweeps_per_bin=1)
     job.update(bad)

@@ -190,6 +190,7 @@ def test_set_coupling_matches_oracle(amd, ctx, orc):
 @pytest.mark.parametrize("kw,msg", [
     (dict(L=12), "power of two"), (dict(L=1), "power of two"), (dict(coupling=-0.44), "coupling"),
     (dict(n_levels=7, L=64), "n_levels"), (dict(sweeps_per_bin=0), "sweeps_per_bin"),
+    (dict(L=4096, sweeps_per_bin=8192), "2^63"),  # max_sweeps_per_bin(4096) = 8191
 ])
 def test_invalid_chain_arguments(amd, ctx, kw, msg):
     args = dict(L=64, coupling=KC, seed=1, n_levels=1)

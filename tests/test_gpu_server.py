@@ -199,7 +199,8 @@ def test_errors_and_cancel(amd, ctx):
                     dict(L=16, coupling=-0.4, seed=1, n_meas=4, sweeps_per_bin=1),
                     dict(L=16, coupling=0.4, seed=1, n_meas=4, n_levels=4, sweeps_per_bin=1),
                     dict(L=16, coupling=0.4, seed=1, n_meas=4, sweeps_per_bin=0),
-                    dict(L=16, coupling=0.4, seed=1, n_meas=4, sweeps_per_bin=1, sampler=1)):
+                    dict(L=16, coupling=0.4, seed=1, n_meas=4, sweeps_per_bin=1, sampler=1),
+                    dict(L=512, coupling=0.4, seed=1, n_meas=4, sweeps_per_bin=40000000)):
             with pytest.raises(amd.MCRGError) as e:
                 srv.submit(bad)
             assert e.value.status == amd.ERR_INVALID_ARG

@@ -155,3 +155,59 @@ def test_tau_int_partial_last_bin_and_errors(amd):
         amd.tau_int(rec[:1], 0, 0, amd.OP_NN)
     with pytest.raises(amd.MCRGError):
         amd.tau_int(rec, 0, 0, 4)
+
+
+def _scaled_bins(orc, rec, nl, c, n_bins):
+    """The total of `rec` with every operator multiplied by the integer c (sums
+    by c, products by c^2), spread over n_bins int64 bins whose exact sum it
+    is.  Scaling the operators is a similarity transform of T (same
+    eigenvalues) and divides the locator step by c."""
+    tot = [int(v) for v in rec.sum(axis=0)]
+    n_s = 3 * (nl + 1)
+    i_m = orc.off_N(nl) - 3  # M, |M|, M^2, count
+    scaled = []
+    for k, v in enumerate(tot):
+        scaled.append(v * c if k < n_s else (v if k >= i_m else v * c * c))
+    out = np.zeros((n_bins, len(tot)), dtype=np.int64)
+    for k, v in enumerate(scaled):
+        q, r = divmod(v, n_bins)
+        assert abs(q) < 2**62
+        out[:, k] = q
+        out[0, k] += r
+    return scaled, out
+
+
+def test_solve_is_exact_beyond_int64_totals(amd, orc, chains):
+    """Bins whose totals pass 2^63 (as the NN self sums of L = 2048 do over 1e6
+    samples) give the same lambda as the unscaled records: the sums over bins
+    are formed in 128-bit integers.  Summed in int64 they would wrap."""
+    L, J, nl = 16, KC, 3
+    rec = chains[(L, J, nl)]
+    c = 1 << 20
+    scaled, bins = _scaled_bins(orc, rec, nl, c, 4096)
+    assert max(abs(v) for v in scaled) > 2**63  # the totals do not fit int64
+    for level in range(nl):
+        for n_ops in (2, 3):
+            want = amd.rg_exponent(rec, nl, level, n_ops)["lambda"]
+            got = amd.rg_exponent(bins, nl, level, n_ops)["lambda"]
+            assert got == pytest.approx(want, rel=1e-12), (level, n_ops)
+    # the locator step: one record of each size, scaled alike
+    _, rl = orc.run_chain(16, 0.44, seed=1, n_therm=300, n_meas=3000, nl=3, sweeps_per_bin=3000)
+    _, rs = orc.run_chain(8, 0.44, seed=2, n_therm=300, n_meas=3000, nl=2, sweeps_per_bin=3000)
+    _, bl = _scaled_bins(orc, rl, 3, c, 512)
+    _, bs = _scaled_bins(orc, rs, 2, c, 256)
+    for level in range(3):
+        want = amd.critical_point_step(rl[0], 3, rs[0], 2, level)
+        assert amd.critical_point_step(bl, 3, bs, 2, level) == pytest.approx(want / c, rel=1e-10)
+
+
+def test_max_sweeps_per_bin_closed_form(amd):
+    """floor((2^63 - 1) / (2 L^2)^2): 2^17 - 1 at L = 2048, 31 at L = 16384."""
+    assert amd.max_sweeps_per_bin(2048) == 2**17 - 1
+    assert amd.max_sweeps_per_bin(4096) == 2**13 - 1
+    assert amd.max_sweeps_per_bin(16384) == 31
+    assert amd.max_sweeps_per_bin(64) == 2**31 - 1  # capped at int32
+    assert amd.max_sweeps_per_bin(1) == 0 and amd.max_sweeps_per_bin(32768) == 0
+    for L in (256, 1024, 8192):
+        m = amd.max_sweeps_per_bin(L)
+        assert m * (2 * L * L) ** 2 < 2**63 <= (m + 1) * (2 * L * L) ** 2
