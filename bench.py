@@ -111,6 +111,25 @@ def run_batched(mcrg_amd, ctx, jobs, steps, warmup, sps, threads, autotune=False
     return sites * sps * steps / (dt * 1e9), dt, sites, opts
 
 
+def autotune_pass(mcrg_amd, ctx, jobs, sps, threads):
+    """Untimed autotuning on the same manifest: launches of `sps` measured
+    sweeps until the tuner has timed every candidate (HIP events on the
+    engine's stream) and kept the fastest; returns the configuration it kept
+    and the launches it took."""
+    n_sweeps = 64 * sps
+    bjobs = [dict(j, n_therm=0, n_meas=n_sweeps, sweeps_per_bin=max(1, n_sweeps)) for j in jobs]
+    with mcrg_amd.Batch(ctx, bjobs, threads=threads, sweeps_per_launch=sps, autotune=True) as b:
+        launches = 0
+        while b.options()["autotune"] and launches < 64:
+            b.run(sps)
+            b.sync()
+            launches += 1
+        opts = b.options()
+    if opts["autotune"]:
+        raise RuntimeError("the autotuner did not finish")
+    return {k: opts[k] for k in ("threads", "min_team", "max_slot_lds", "threads_large", "sub_chunks")}, launches
+
+
 def run_batched_timed(mcrg_amd, ctx, jobs, sps, tuned, launches):
     """Kernel duration of the resident kernel from HIP events on its stream,
     in the configuration `tuned` (the one the timed leg ran)."""
@@ -131,19 +150,28 @@ def run_batched_timed(mcrg_amd, ctx, jobs, sps, tuned, launches):
 
 
 def run_plain_mixed(mcrg_amd, ctx, jobs, sweeps):
-    """Plain path, one replica at a time: updates/ns over the whole sample, and
-    each replica's records."""
-    updates = 0
-    recs = []
-    t0 = time.perf_counter()
-    for j in jobs:
-        with mcrg_amd.Chain(ctx, j["L"], j["coupling"], j["seed"], n_levels=j["n_levels"],
-                            sweeps_per_bin=sweeps, max_bins=1) as ch:
+    """Plain path, one replica at a time: every chain is created, initialised
+    and warmed (graphs captured) before the clock starts; the timed region is
+    each chain's `sweeps` measured sweeps in turn, each run to completion.
+    Records are read after it.  Returns updates/ns, seconds and the records."""
+    chains = []
+    try:
+        for j in jobs:
+            ch = mcrg_amd.Chain(ctx, j["L"], j["coupling"], j["seed"], n_levels=j["n_levels"],
+                                sweeps_per_bin=2 * sweeps, max_bins=1)
+            chains.append(ch)
+            ch.sweep(sweeps, measure=True)  # warm-up: captures the replayed graph, loads the kernels
+            ch.sync()
+        updates = sum(j["L"] * j["L"] * sweeps for j in jobs)
+        t0 = time.perf_counter()
+        for ch in chains:
             ch.sweep(sweeps, measure=True)
             ch.sync()
-            recs.append(ch.records())
-        updates += j["L"] * j["L"] * sweeps
-    dt = time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        recs = [ch.records() for ch in chains]
+    finally:
+        for ch in chains:
+            ch.close()
     return updates / (dt * 1e9), dt, recs
 
 
@@ -162,7 +190,9 @@ def matched_yt(mcrg_amd, ctx, jobs, sweeps, plain_recs):
 def run_plain(mcrg_amd, ctx, L, nl, steps, measure=True, warm=36):
     """One plain int8 chain (|K| = 0.4406868, seed 7): `warm` sweeps (which
     also capture the replayed graph), then `steps` timed sweeps."""
-    with mcrg_amd.Chain(ctx, L, 0.4406868, 7, n_levels=nl, sweeps_per_bin=steps + warm, max_bins=1) as ch:
+    spb = max(1, min(steps + warm, mcrg_amd.max_sweeps_per_bin(L)))  # int64-safe bins (L = 16384: 31 sweeps)
+    with mcrg_amd.Chain(ctx, L, 0.4406868, 7, n_levels=nl, sweeps_per_bin=spb,
+                        max_bins=-(-(steps + warm) // spb)) as ch:
         ch.sweep(warm, measure=measure)
         ch.sync()
         t0 = time.perf_counter()
@@ -248,7 +278,31 @@ def _serve(mcrg_amd, srv, stream, rate, duration, drain_limit, quantum):
             "launch_quantum_us": quantum or 4000}
 
 
+def host_cpu():
+    """What the host reports about its cores."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return {"model": model, "os_cpu_count": os.cpu_count(), "affinity_cpus": affinity}
+
+
 def cpu_baseline(jobs, sweeps, threads):
+    """The oracle (the same Philox Metropolis, blocking and int64 records in
+    scalar C) on the host's cores:
+      value       threads over independent replicas of the batching manifest
+      one_thread  the plain configurations on one thread: L = 8 in full
+                  (1e4 + 1e5 sweeps, one blocking, seed 1) and the first 100
+                  sweeps of L = 2048 (seed 7)"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
     orc.lib()
@@ -261,19 +315,41 @@ def cpu_baseline(jobs, sweeps, threads):
     with cf.ThreadPoolExecutor(max_workers=threads) as ex:  # ctypes releases the GIL
         updates = sum(ex.map(one, jobs))
     dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    orc.run_chain(8, 0.4406868, 1, 10000, 100000, 1, 10000)
+    dt1 = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    orc.run_chain(2048, 0.4406868, 7, 100, 0, 5, 1)
+    dt2 = time.perf_counter() - t2
+    host = host_cpu()
     return {"value": updates / (dt * 1e9), "unit": "spin-updates/ns", "cores": threads, "kind": "port",
-            "sample": f"oracle (scalar C, -O3) on {len(jobs)} replicas of the batching manifest x {sweeps} "
-                      f"measured sweeps each, {threads} threads over independent replicas: {dt:.1f} s"}
+            "sample": f"oracle (scalar C, -O3 -march=x86-64-v3) on {len(jobs)} replicas of the batching manifest x "
+                      f"{sweeps} measured sweeps each, {threads} threads over independent replicas: {dt:.1f} s",
+            "host": dict(host, threads_used=threads),
+            "one_thread": {
+                "L8_full": {"updates_per_ns": 64 * 110000 / (dt1 * 1e9), "seconds": dt1,
+                            "sample": "L=8, |K|=0.4406868, seed 1, 1e4 + 1e5 sweeps, every measured sweep "
+                                      "blocked once (8->4), int64 records"},
+                "L2048_first_100": {"updates_per_ns": 2048 * 2048 * 100 / (dt2 * 1e9), "seconds": dt2,
+                                    "sample": "L=2048, |K|=0.4406868, seed 7, sweeps 1..100 (unmeasured)"}}}
 
 
-def compute_line(cfg_key, sites, sps):
-    m = MEASURED_VALU.get(cfg_key)
+def valu_roofline(cfg_key, sites, sps, avg_ms, clock_khz, cus):
+    """VALU issue roof of the resident kernel: wave64 VALU instructions per
+    launch (rocprofv3 SQ_INSTS_VALU) x 64 lanes over the launch's event-timed
+    duration, against cus x 4 SIMDs x 16 lanes x the engine clock."""
+    m = MEASURED_VALU.get(cfg_key) or MEASURED_VALU.get(cfg_key[:-1] + (1,))
     if not m:
         return None
-    capacity = 1024 * m["gui_active_cycles_per_xcd"] / 4.0
-    return {"bound": "valu", "frac": m["valu_wave_instr_per_launch"] / capacity,
+    peak = cus * 4 * 16 * clock_khz * 1e3 / 1e12  # T lane-instructions / s
+    achieved = 64.0 * m["valu_wave_instr_per_launch"] / (avg_ms * 1e-3) / 1e12
+    exact = cfg_key in MEASURED_VALU
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "T lane-instr/s", "frac": achieved / peak,
             "lane_instr_per_update": 64.0 * m["valu_wave_instr_per_launch"] / (sites * sps),
-            "source": "rocprofv3 --pmc SQ_INSTS_VALU GRBM_GUI_ACTIVE on bench.py --only batched"}
+            "source": ("rocprofv3 --pmc SQ_INSTS_VALU on bench.py --only batched "
+                       + ("(this configuration)" if exact else "with every slot in one part (the parts of this "
+                          "configuration add only their reloads)"))
+                      + f"; duration {avg_ms:.4f} ms from HIP events on the kernel's stream in this run"}
 
 
 def run_bisection(rounds):
@@ -317,9 +393,11 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--serving-rates", default="200,1000",
+    ap.add_argument("--serving-rates", default="200,1000,5000",
                     help="comma-separated arrival rates in jobs/s for the serving leg ('' = none)")
-    ap.add_argument("--serving-duration", type=float, default=10.0)
+    ap.add_argument("--serving-duration", type=float, default=60.0)
+    ap.add_argument("--plain-sweeps", type=int, default=64, help="timed measured sweeps per plain_mixed chain")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the untimed autotuning pass")
     ap.add_argument("--serving-drain", type=float, default=120.0,
                     help="seconds after the last arrival to wait for completions")
     ap.add_argument("--serving-quantum", type=int, default=0)
@@ -379,8 +457,17 @@ def main():
     dev = ctx.device_info()
     log(f"bench: {dev['name']} {dev['compute_units']} CUs; {len(jobs)} replicas, "
         f"{args.warmup}+{args.steps} steps x {args.sweeps_per_step} sweeps")
+    autotuned = {"ran": False}
+    cfg = dict(min_team=16, max_slot_lds=0, threads_large=0, sub_chunks=0)
+    threads = args.threads
+    if not args.no_autotune:
+        chosen, tune_launches = autotune_pass(mcrg_amd, ctx, jobs, args.sweeps_per_step, args.threads)
+        threads = chosen.pop("threads")
+        cfg = chosen
+        autotuned = {"ran": True, "launches": tune_launches, "kept": dict(cfg, threads=threads)}
+        log(f"bench: autotuner kept threads={threads} {cfg} after {tune_launches} untimed launches")
     value, dt, sites, tuned = run_batched(mcrg_amd, ctx, jobs, args.steps, args.warmup, args.sweeps_per_step,
-                                          args.threads, autotune=args.warmup >= 10)
+                                          threads, cfg=cfg)
     log(f"bench: batched {value:.1f} updates/ns ({dt:.3f} s timed); configuration "
         f"threads={tuned['threads']} min_team={tuned['min_team']} max_slot_lds={tuned['max_slot_lds']} "
         f"threads_large={tuned['threads_large']} sub_chunks={tuned['sub_chunks']}")
@@ -390,17 +477,19 @@ def main():
                tuned["threads_large"], tuned["sub_chunks"])
     bpl = bytes_per_update(True) * sites * args.sweeps_per_step
     achieved = bpl / (avg_ms * 1e-3) / 1e9
-    log(f"bench: resident kernel {avg_ms:.3f} ms/launch over {n_launch} launches -> {achieved:.1f} GB/s")
+    log(f"bench: resident kernel {avg_ms:.3f} ms/launch over {n_launch} launches -> {achieved:.1f} GB/s algorithmic")
 
     plain_mixed = plain_2048 = matched = None
     if not args.no_plain:
         sample = jobs[:args.plain_sample]
-        plain_mixed, pdt, plain_recs = run_plain_mixed(mcrg_amd, ctx, sample, 20)
-        log(f"bench: plain path on {len(sample)} replicas x 20 sweeps: {plain_mixed:.3f} updates/ns")
-        matched = matched_yt(mcrg_amd, ctx, sample, 20, plain_recs)
+        plain_mixed, pdt, plain_recs = run_plain_mixed(mcrg_amd, ctx, sample, args.plain_sweeps)
+        log(f"bench: plain path on {len(sample)} replicas x {args.plain_sweeps} sweeps: {plain_mixed:.3f} updates/ns "
+            f"({pdt:.3f} s timed)")
+        matched = matched_yt(mcrg_amd, ctx, sample, 2 * args.plain_sweeps, plain_recs)
         log(f"bench: batched records of the same replicas identical: {matched['records_identical']}")
         plain_2048 = run_plain_2048(mcrg_amd, ctx, 500)
         log(f"bench: plain L=2048: {plain_2048:.1f} updates/ns")
+    valu = valu_roofline(cfg_key, sites, args.sweeps_per_step, avg_ms, dev["clock_khz"], dev["compute_units"])
     serving = []
     for rate in [float(r) for r in args.serving_rates.split(",") if r.strip()]:
         res = run_serving(mcrg_amd, workloads, ctx, rate, args.serving_duration, args.serving_drain,
@@ -412,9 +501,11 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # every core this process may run on, at most 16 (one GPU's share of a shared host)
+        host = host_cpu()
+        threads = max(1, min(16, host["affinity_cpus"] or os.cpu_count() or 1))
         sample = jobs[:1024]
-        log(f"bench: cpu baseline on {len(sample)} replicas x {args.cpu_sweeps} sweeps, {threads} threads")
+        log(f"bench: cpu baseline on {len(sample)} replicas x {args.cpu_sweeps} sweeps, {threads} threads ({host})")
         cpu = cpu_baseline(sample, args.cpu_sweeps, threads)
 
     out = {
@@ -433,15 +524,18 @@ def main():
         "config": {"workload": f"batched: {len(jobs)} replicas, L in {{8..128}}, 5 couplings, seeds 0..{len(jobs)-1}, "
                                f"bit-packed LDS-resident, blocking to L=4, every sweep measured",
                    "replicas": len(jobs), "sweeps_per_step": args.sweeps_per_step, "threads": tuned["threads"],
-                   "autotuned": {k: tuned[k] for k in ("threads", "min_team", "max_slot_lds", "threads_large",
-                                                       "sub_chunks")},
+                   "autotuned": autotuned,
+                   "run_config": {k: tuned[k] for k in ("threads", "min_team", "max_slot_lds", "threads_large",
+                                                        "sub_chunks")},
                    "sites": sites, "layout": "packed", "schedule": "resident", "device": dev["name"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get(cfg_key),
-                     "kernel": f"k_resident<{tuned['threads']}>",
-                     "avg_ms": avg_ms, "bytes_per_launch": bpl},
-        "compute": compute_line(cfg_key, sites, args.sweeps_per_step),
+        "roofline": dict(valu or {"bound": "valu", "achieved": None, "peak": None, "unit": "T lane-instr/s",
+                                  "frac": None, "source": "no counter run for this configuration"},
+                         traffic=MEASURED_TRAFFIC.get(cfg_key), kernel=f"k_resident<{tuned['threads']}>",
+                         avg_ms=avg_ms,
+                         hbm={"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": achieved / HBM_PEAK_GBS, "bytes_per_launch": bpl,
+                              "basis": "algorithmic bytes (3/8 B per site update + 5/24 B per site per measured "
+                                       "sweep, bit-packed streaming model)"}),
         "plain_mixed": plain_mixed,
         "plain_l2048": plain_2048,
         "batched_over_plain_mixed": (value / plain_mixed) if plain_mixed else None,

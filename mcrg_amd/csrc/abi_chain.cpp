@@ -6,6 +6,7 @@
 
 #include "hbm_group.hpp"
 #include "runtime.hpp"
+#include "strip.hpp"
 
 namespace mcrg {
 
@@ -96,7 +97,20 @@ struct mcrg_chain_s {
     uint32_t t = 0;                // host mirror of state[0]
     uint32_t n_measured = 0;       // host mirror of state[1]
     SweepRunner runner;            // replayed graphs of long sweep runs
+    StripRunner strip;             // cooperative strip launches (large Metropolis chains)
 };
+
+// Fails when a strip launch since the last check gave up on a wait.
+static mcrg_status check_strip(mcrg_chain ch)
+{
+    bool bad = false;
+    MCRG_HIP(ch->strip.failed(ch->stream, bad));
+    if (bad) {
+        set_error("strip launch: a workgroup's wait for its neighbours exceeded the limit; the chain state is lost");
+        return MCRG_ERR_DEVICE;
+    }
+    return MCRG_OK;
+}
 
 extern "C" {
 
@@ -291,7 +305,12 @@ mcrg_status mcrg_chain_sweep(mcrg_chain ch, int32_t n_sweeps, int32_t measure)
     MCRG_REQUIRE((uint64_t)ch->t + (uint64_t)n_sweeps <= 0xFFFFFFFFull, "sweep counter overflow");
     MCRG_HIP(hipSetDevice(ch->ctx->device));
     KernelTimer *tm = ch->ctx->timer.enabled ? &ch->ctx->timer : nullptr;
-    MCRG_HIP(ch->runner.run(ch->d_desc, 1, d.L, d.n_levels, d.sampler, (uint32_t)n_sweeps, measure != 0, ch->stream, tm));
+    if (ch->strip.usable(ch->host, measure != 0))
+        MCRG_HIP(ch->strip.run(ch->d_desc, ch->host, ch->t, ch->n_measured, (uint32_t)n_sweeps, measure != 0,
+                               ch->stream, tm));
+    else
+        MCRG_HIP(ch->runner.run(ch->d_desc, 1, d.L, d.n_levels, d.sampler, (uint32_t)n_sweeps, measure != 0,
+                                ch->stream, tm));
     ch->t += (uint32_t)n_sweeps;
     if (measure) ch->n_measured += (uint32_t)n_sweeps;
     return MCRG_OK;
@@ -302,7 +321,7 @@ mcrg_status mcrg_chain_sync(mcrg_chain ch)
     MCRG_REQUIRE(ch != nullptr, "chain is null");
     MCRG_HIP(hipSetDevice(ch->ctx->device));
     MCRG_HIP(hipStreamSynchronize(ch->stream));
-    return MCRG_OK;
+    return check_strip(ch);
 }
 
 mcrg_status mcrg_chain_get_spins(mcrg_chain ch, int8_t *host, int64_t n)
@@ -312,7 +331,7 @@ mcrg_status mcrg_chain_get_spins(mcrg_chain ch, int8_t *host, int64_t n)
     MCRG_HIP(hipSetDevice(ch->ctx->device));
     MCRG_HIP(hipMemcpyAsync(host, ch->host.spins, (size_t)n, hipMemcpyDeviceToHost, ch->stream));
     MCRG_HIP(hipStreamSynchronize(ch->stream));
-    return MCRG_OK;
+    return check_strip(ch);
 }
 
 mcrg_status mcrg_chain_set_spins(mcrg_chain ch, const int8_t *host, int64_t n)
@@ -335,7 +354,7 @@ mcrg_status mcrg_chain_get_records(mcrg_chain ch, int64_t *host, int64_t n)
     MCRG_HIP(hipSetDevice(ch->ctx->device));
     MCRG_HIP(hipMemcpyAsync(host, ch->host.records, (size_t)n * 8, hipMemcpyDeviceToHost, ch->stream));
     MCRG_HIP(hipStreamSynchronize(ch->stream));
-    return MCRG_OK;
+    return check_strip(ch);
 }
 
 mcrg_status mcrg_chain_get_state(mcrg_chain ch, uint32_t *sweep, uint32_t *n_measured)

@@ -160,13 +160,21 @@ static mcrg_status server_step(mcrg_server s, int32_t *active)
             done.push_back(std::move(d));
         }
     }
-    for (Done &d : done) {
-        e = s->engine.read_records(d.engine_id, d.rec.data(), (int64_t)d.rec.size());
+    if (!done.empty()) {
+        std::vector<int> ids;
+        std::vector<int64_t *> dst;
+        std::vector<int64_t> ns;
+        for (Done &d : done) {
+            ids.push_back(d.engine_id);
+            dst.push_back(d.rec.data());
+            ns.push_back((int64_t)d.rec.size());
+        }
+        e = s->engine.read_records_many(ids, dst, ns);
         if (e != hipSuccess) {
             set_error(std::string("server readback: ") + hipGetErrorString(e));
             return MCRG_ERR_DEVICE;
         }
-        s->engine.remove(d.engine_id);
+        for (Done &d : done) s->engine.remove(d.engine_id);
     }
     {
         std::lock_guard<std::mutex> lk(s->mu);

@@ -122,6 +122,56 @@ __device__ __forceinline__ uint32_t metropolis_word(uint32_t s, uint32_t nu, uin
     return (flip | accept) & valid;
 }
 
+// The two Philox calls metropolis_word draws first for a word (the 4 top bits
+// of every site's uniform, and the first refinement call), which depend only
+// on the word, sweep and colour: a caller can draw them ahead of the spins.
+__device__ __forceinline__ void metropolis_rng(uint32_t word, uint32_t t, uint32_t colour, uint32_t k0, uint32_t k1,
+                                               u32x4 &r, u32x4 &ref)
+{
+    const uint32_t tag = (kTagSweep << 28) | (colour << 24);
+    r = philox(word, t, 0u, tag, k0, k1);
+    ref = philox(word, t, 1u, tag, k0, k1);
+}
+
+// metropolis_word with the first two Philox calls drawn by metropolis_rng
+// (the same decisions bit for bit).
+__device__ __forceinline__ uint32_t metropolis_word_drawn(uint32_t s, uint32_t nu, uint32_t nd, uint32_t nl,
+                                                          uint32_t nr, const Accept &acc, uint32_t k0, uint32_t k1,
+                                                          uint32_t word, uint32_t t, uint32_t colour,
+                                                          const u32x4 &r, u32x4 ref)
+{
+    const uint32_t a1 = s ^ nu, a2 = s ^ nd, a3 = s ^ nl, a4 = s ^ nr;
+    const uint32_t x1 = a1 ^ a2, y1 = a1 & a2, x2 = a3 ^ a4, y2 = a3 & a4;
+    const uint32_t any = a1 | a2 | a3 | a4;
+    const uint32_t c1 = (x1 ^ x2) & ~(y1 | y2);
+    const uint32_t c2 = ~any;
+    uint32_t flip = any & ~c1;
+    flip |= (acc.always0 ? c1 : 0u) | (acc.always1 ? c2 : 0u);
+    const uint32_t m1 = acc.always0 ? 0u : c1;
+    const uint32_t m2 = acc.always1 ? 0u : c2;
+    uint32_t open = m1 | m2;
+    uint32_t accept = 0;
+    const uint32_t tag = (kTagSweep << 28) | (colour << 24);
+    const uint32_t rq[4] = {r.v0, r.v1, r.v2, r.v3};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t b0 = 0u - ((acc.thr0 >> (31 - q)) & 1u);
+        const uint32_t b1 = 0u - ((acc.thr1 >> (31 - q)) & 1u);
+        const uint32_t tb = (m1 & b0) | (m2 & b1);
+        accept |= open & ~rq[q] & tb;
+        open &= ~(rq[q] ^ tb);
+    }
+    for (uint32_t rank = 0; open != 0u; ++rank) {
+        const uint32_t i = (uint32_t)__builtin_ctz(open);
+        open &= open - 1u;
+        if (rank != 0u && (rank & 3u) == 0u) ref = philox(word, t, 1u + (rank >> 2), tag, k0, k1);
+        const uint32_t low = pick4(ref, rank & 3u) >> 4;
+        const uint32_t thr = ((m1 >> i) & 1u) ? acc.thr0 : acc.thr1;
+        accept |= (low < (thr & 0x0FFFFFFFu) ? 1u : 0u) << i;
+    }
+    return flip | accept;
+}
+
 // Spin bits of the 32 bytes at even (p = 0) or odd (p = 1) positions of a 64-byte
 // chunk held as 16 dwords: result bit j = sign bit of byte 2j + p.
 __device__ __forceinline__ uint32_t gather_parity(const uint32_t d[16], uint32_t p)

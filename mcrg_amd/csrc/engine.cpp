@@ -92,6 +92,10 @@ ResidentEngine::~ResidentEngine()
     if (d_done_) (void)hipFree(d_done_);
     if (d_err_) (void)hipFree(d_err_);
     if (d_init_idx_) (void)hipFree(d_init_idx_);
+    if (d_stage_) (void)hipFree(d_stage_);
+    if (h_stage_) (void)hipHostFree(h_stage_);
+    if (h_stage_idx_) (void)hipHostFree(h_stage_idx_);
+    if (h_rec_) (void)hipHostFree(h_rec_);
 }
 
 hipError_t ResidentEngine::init(int device, size_t arena_bytes, int capacity, const EngineOptions &opt,
@@ -109,6 +113,7 @@ hipError_t ResidentEngine::init(int device, size_t arena_bytes, int capacity, co
     if ((e = hipMemset(d_err_, 0, sizeof(uint32_t))) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) return e;
     if ((e = hipMalloc(&d_init_idx_, sizeof(int32_t) * (size_t)capacity)) != hipSuccess) return e;
+    if ((e = hipMalloc(&d_stage_, sizeof(PackedLattice) * (size_t)capacity)) != hipSuccess) return e;
     for (hipStream_t &a : aux_)
         if ((e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking)) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming)) != hipSuccess) return e;
@@ -281,13 +286,30 @@ hipError_t ResidentEngine::flush_pending()
 {
     if (pending_init_.empty()) return hipSuccess;
     hipError_t e;
-    // descriptors of the new lattices, then one init launch for all of them
-    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
-    staging_lats_.resize(pending_init_.size());
-    for (size_t k = 0; k < pending_init_.size(); ++k) {
+    // descriptors of the new lattices staged in pinned memory, two copies and
+    // one admission launch for all of them (stream order puts it after any
+    // launch that still reads the table; new ids are in no slot)
+    const size_t n = pending_init_.size();
+    if (n > stage_cap_) {
+        if (h_stage_) (void)hipHostFree(h_stage_);
+        if (h_stage_idx_) (void)hipHostFree(h_stage_idx_);
+        h_stage_ = nullptr;
+        h_stage_idx_ = nullptr;
+        stage_cap_ = 0;
+        const size_t cap = std::max<size_t>(n, 256);
+        if ((e = hipHostMalloc(reinterpret_cast<void **>(&h_stage_), cap * sizeof(PackedLattice),
+                               hipHostMallocDefault)) != hipSuccess)
+            return e;
+        if ((e = hipHostMalloc(reinterpret_cast<void **>(&h_stage_idx_), cap * sizeof(int32_t),
+                               hipHostMallocDefault)) != hipSuccess)
+            return e;
+        stage_cap_ = cap;
+    }
+    for (size_t k = 0; k < n; ++k) {
         const int id = pending_init_[k];
         const Job &job = jobs_[(size_t)id];
-        PackedLattice &P = staging_lats_[k];
+        h_stage_idx_[k] = id;
+        PackedLattice &P = h_stage_[k];
         std::memset(&P, 0, sizeof(P));
         const uint32_t W = cb_words((uint32_t)job.spec.L);
         P.spins = reinterpret_cast<uint32_t *>(arena_.base() + job.off);
@@ -303,17 +325,15 @@ hipError_t ResidentEngine::flush_pending()
         P.n_levels = job.spec.n_levels;
         P.sweeps_per_bin = job.spec.sweeps_per_bin;
         P.max_bins = job.n_bins;
-        if ((e = hipMemcpyAsync(d_lats_ + id, &P, sizeof(P), hipMemcpyHostToDevice, stream_)) != hipSuccess)
-            return e;
-        const size_t rec_bytes = (size_t)job.n_bins * record_size((uint32_t)job.spec.n_levels) * 8u;
-        if ((e = hipMemsetAsync(P.records, 0, rec_bytes, stream_)) != hipSuccess) return e;
     }
-    if ((e = hipMemcpyAsync(d_init_idx_, pending_init_.data(), sizeof(int32_t) * pending_init_.size(),
-                            hipMemcpyHostToDevice, stream_)) != hipSuccess)
+    if ((e = hipMemcpyAsync(d_stage_, h_stage_, sizeof(PackedLattice) * n, hipMemcpyHostToDevice, stream_)) !=
+        hipSuccess)
         return e;
-    if ((e = launch_packed_init(d_lats_, d_init_idx_, (int)pending_init_.size(), stream_)) != hipSuccess)
+    if ((e = hipMemcpyAsync(d_init_idx_, h_stage_idx_, sizeof(int32_t) * n, hipMemcpyHostToDevice, stream_)) !=
+        hipSuccess)
         return e;
-    // the copies above read host memory that is reused later
+    if ((e = launch_packed_init(d_lats_, d_stage_, d_init_idx_, (int)n, stream_)) != hipSuccess) return e;
+    // the pinned staging is rewritten by the next admission
     if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
     pending_init_.clear();
     return hipSuccess;
@@ -629,6 +649,42 @@ hipError_t ResidentEngine::read_spins(int id, int8_t *host)
                 const uint32_t x = 2u * k + ((y + c) & 1u);
                 host[y * L + x] = ((words[c * W + w] >> b) & 1u) ? -1 : 1;
             }
+    return hipSuccess;
+}
+
+hipError_t ResidentEngine::read_records_many(const std::vector<int> &ids, const std::vector<int64_t *> &hosts,
+                                             const std::vector<int64_t> &ns)
+{
+    hipError_t e;
+    if ((e = flush_pending()) != hipSuccess) return e;
+    size_t total = 0;
+    for (int64_t n : ns) total += (size_t)n;
+    if (total > rec_cap_) {
+        if (h_rec_) (void)hipHostFree(h_rec_);
+        h_rec_ = nullptr;
+        rec_cap_ = 0;
+        const size_t cap = std::max<size_t>(total, 1u << 16);
+        if ((e = hipHostMalloc(reinterpret_cast<void **>(&h_rec_), cap * 8u, hipHostMallocDefault)) != hipSuccess)
+            return e;
+        rec_cap_ = cap;
+    }
+    // every copy queued into pinned memory, one wait for all of them
+    size_t off = 0;
+    for (size_t k = 0; k < ids.size(); ++k) {
+        const Job &job = jobs_[(size_t)ids[k]];
+        const uint32_t W = cb_words((uint32_t)job.spec.L);
+        const char *rec = arena_.base() + job.off + round_up(2u * W * 4u, 256);
+        if ((e = hipMemcpyAsync(h_rec_ + off, rec, (size_t)ns[k] * 8u, hipMemcpyDeviceToHost, stream_)) != hipSuccess)
+            return e;
+        off += (size_t)ns[k];
+    }
+    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
+    if ((e = check_parts()) != hipSuccess) return e;
+    off = 0;
+    for (size_t k = 0; k < ids.size(); ++k) {
+        std::memcpy(hosts[k], h_rec_ + off, (size_t)ns[k] * 8u);
+        off += (size_t)ns[k];
+    }
     return hipSuccess;
 }
 

@@ -113,6 +113,9 @@ class ResidentEngine {
 
     hipError_t read_spins(int id, int8_t *host);  // row-major +-1 (synchronous)
     hipError_t read_records(int id, int64_t *host, int64_t n);
+    // Records of several lattices through one pinned staging buffer and one wait.
+    hipError_t read_records_many(const std::vector<int> &ids, const std::vector<int64_t *> &hosts,
+                                 const std::vector<int64_t> &ns);
     void state(int id, uint32_t &t, uint32_t &n_measured) const;
     const JobSpec &spec(int id) const { return jobs_[id].spec; }
     int n_bins(int id) const { return jobs_[id].n_bins; }
@@ -214,7 +217,12 @@ class ResidentEngine {
     bool slots_dirty_ = true;
     uint32_t slots_chunk_ = 0;
     std::vector<int32_t> pending_init_;
-    std::vector<PackedLattice> staging_lats_;
+    // admission staging (pinned) and its device copy; record readback staging
+    PackedLattice *h_stage_ = nullptr, *d_stage_ = nullptr;
+    int32_t *h_stage_idx_ = nullptr;
+    size_t stage_cap_ = 0;
+    int64_t *h_rec_ = nullptr;
+    size_t rec_cap_ = 0;
     const char *failed_ = "";
 };
 

@@ -463,29 +463,6 @@ __device__ void rm_block(const uint32_t *src, uint32_t Ll, uint32_t *dst, uint32
         dst[i] = rm_block_item(src, Ll, i, tie_coin_word(i, t, level, k0, k1));
 }
 
-// Entry j of the per-sample contribution to a bin record (layout of off_S,
-// off_self, off_cross, off_nn0, off_M) from the operator values S[l][a] and
-// the level-0 magnetisation M.
-__device__ __forceinline__ int64_t record_term(uint32_t j, uint32_t nl, const int64_t (*S)[3], int64_t M)
-{
-    const uint32_t n_s = 3u * (nl + 1u), n_self = 9u * (nl + 1u), n_cross = 9u * nl;
-    if (j < n_s) return S[j / 3u][j % 3u];
-    j -= n_s;
-    if (j < n_self) {
-        const uint32_t l = j / 9u, a = j % 3u, b = (j % 9u) / 3u;
-        return S[l][a] * S[l][b];
-    }
-    j -= n_self;
-    if (j < n_cross) {
-        const uint32_t l = j / 9u + 1u, a = j % 3u, b = (j % 9u) / 3u;
-        return S[l][a] * S[l - 1u][b];
-    }
-    j -= n_cross;
-    if (j < nl) return S[j + 1u][0] * S[0][0];
-    j -= nl;
-    return j == 0u ? M : j == 1u ? (M < 0 ? -M : M) : j == 2u ? M * M : 1;
-}
-
 // One workgroup per lattice: the levels below kGridLevelMinL (the first read
 // from global memory, the deeper ones in LDS), then the operator values of
 // every level (grid-measured levels and level 0 from the counters, which it

@@ -80,7 +80,10 @@ __host__ __device__ inline SlotLayout slot_layout(uint32_t L, uint32_t G, uint32
     return s;
 }
 
-hipError_t launch_packed_init(PackedLattice *lats, const int32_t *idx, int n, hipStream_t s);
+// Admits n lattices: lats[idx[k]] = stage[k], its records zeroed, its spins
+// drawn (stage and idx in device memory).
+hipError_t launch_packed_init(PackedLattice *lats, const PackedLattice *stage, const int32_t *idx, int n,
+                              hipStream_t s);
 // Work items of one launch: sub-chunk k (0 <= k < n_sub) of slot i is item
 // k * n_slots + i, so that every slot's first part is queued before any second
 // part.  Sub-chunk k runs sweeps [k c, min((k+1) c, n_sweeps)) of its slot,

@@ -1,7 +1,8 @@
 // LDS-resident schedule kernels (see resident_core.hpp for the phase structure).
 //
-//   k_packed_init   Philox +-1 start of bit-packed (CB) lattices
-//                   (Lattice::initialize_random_spins, src/lattice.cpp:33-41)
+//   k_packed_init   admission of bit-packed (CB) lattices: descriptor from a
+//                   staging array into the table, records zeroed, Philox +-1
+//                   start (Lattice::initialize_random_spins, src/lattice.cpp:33-41)
 //   k_resident      persistent grid; each workgroup pulls work items (one part
 //                   of one slot: G lattices of one L) from an atomic queue and
 //                   runs the part's sweeps and measurements with the lattices
@@ -17,12 +18,16 @@
 namespace mcrg {
 
 __global__ __launch_bounds__(256) void k_packed_init(PackedLattice *__restrict__ lats,
+                                                     const PackedLattice *__restrict__ stage,
                                                      const int32_t *__restrict__ idx)
 {
-    PackedLattice &P = lats[idx[blockIdx.y]];
+    const PackedLattice P = stage[blockIdx.y];
     const uint32_t L = (uint32_t)P.L;
     const uint32_t W = cb_words(L);
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == 0u) lats[idx[blockIdx.y]] = P;
+    const uint64_t n_rec = (uint64_t)P.max_bins * record_size((uint32_t)P.n_levels);
+    for (uint64_t i = j; i < n_rec; i += (uint64_t)gridDim.x * blockDim.x) P.records[i] = 0;
     if (j >= 2u * W) return;
     const uint32_t c = j / W, w = j % W;
     const uint32_t half = L >> 1;
@@ -130,12 +135,13 @@ __global__ void k_reset_queue(uint32_t *queue, uint32_t *done, uint32_t n)
     if (i < n) done[i] = 0u;
 }
 
-hipError_t launch_packed_init(PackedLattice *lats, const int32_t *idx, int n, hipStream_t s)
+hipError_t launch_packed_init(PackedLattice *lats, const PackedLattice *stage, const int32_t *idx, int n,
+                              hipStream_t s)
 {
     // every lattice in the list is initialised with its own L; the grid covers L = 512
     const uint32_t max_words = 2u * cb_words((uint32_t)kResidentMaxL);
     hipLaunchKernelGGL(k_packed_init, dim3((max_words + 255u) / 256u, (uint32_t)n), dim3(256), 0, s,
-                       lats, idx);
+                       lats, stage, idx);
     return hipGetLastError();
 }
 

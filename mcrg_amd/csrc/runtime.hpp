@@ -47,7 +47,7 @@ Accept make_accept(double coupling);
 // Swendsen-Wang bond threshold floor((1 - exp(-2J)) 2^32).
 uint32_t bond_threshold(double coupling);
 
-enum KernelKind { kKindSweep = 0, kKindMeasure0 = 1, kKindLevels = 2, kKindResident = 3, kNumKinds = 4 };
+enum KernelKind { kKindSweep = 0, kKindMeasure0 = 1, kKindLevels = 2, kKindResident = 3, kKindStrip = 4, kNumKinds = 5 };
 
 // Event-pair timing of kernel launches, per context.
 struct KernelTimer {
@@ -58,8 +58,8 @@ struct KernelTimer {
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
-    int64_t launches[kNumKinds] = {0, 0, 0, 0};
-    double total_ms[kNumKinds] = {0, 0, 0, 0};
+    int64_t launches[kNumKinds] = {0, 0, 0, 0, 0};
+    double total_ms[kNumKinds] = {0, 0, 0, 0, 0};
     std::mutex mu;
 
     hipEvent_t get();

@@ -1,0 +1,117 @@
+// Host side of the strip schedule (strip.hpp).
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+
+#include "runtime.hpp"
+#include "strip.hpp"
+
+namespace mcrg {
+
+StripRunner::~StripRunner()
+{
+    if (ws_) (void)hipFree(ws_);
+}
+
+static bool strip_enabled()
+{
+    const char *v = std::getenv("MCRG_STRIP");
+    return !(v && std::strcmp(v, "0") == 0);
+}
+
+bool StripRunner::plan(const HbmLattice &h, bool measure, StripShape &sh, int &threads)
+{
+    if (h.sampler != MCRG_SAMPLER_METROPOLIS || !strip_enabled()) return false;
+    const uint32_t L = (uint32_t)h.L;
+    if (L < 64u || L > (measure ? 8192u : 16384u)) return false;
+    if (cus_ < 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus_ = 0;
+    }
+    uint32_t n_wg = 1;
+    while (2u * n_wg <= (uint32_t)cus_ && 2u * n_wg <= kStripMaxWg) n_wg *= 2u;
+    if (const char *v = std::getenv("MCRG_STRIP_WG")) {  // fewer strips (a power of two) when asked
+        const uint32_t want = (uint32_t)std::max(2, std::atoi(v));
+        while (n_wg > want) n_wg /= 2u;
+    }
+    if (n_wg > L / 2u) n_wg = L / 2u;
+    if (n_wg < 2u) return false;
+    // tail workgroups: 4 (fewer when they do not fit beside the strips);
+    // MCRG_STRIP_TAILS = 1, 2 or 4 asks for that many
+    uint32_t want = 4u;
+    if (const char *v = std::getenv("MCRG_STRIP_TAILS")) {
+        const int n = std::atoi(v);
+        if (n == 1 || n == 2 || n == 4) want = (uint32_t)n;
+    }
+    for (uint32_t n_tail : {4u, 2u, 1u}) {
+        if (n_tail > want) continue;
+        sh = strip_shape(L, (uint32_t)h.n_levels, n_wg, n_tail, measure);
+        threads = strip_threads(sh);
+        // every thread gets at most one neighbour word of each kind per exchange
+        if (2u * sh.wpr > (uint32_t)threads || sh.first_words > (uint32_t)threads) return false;
+        const uint32_t lds = sh.lds_words * 4u;
+        if (lds > 160u * 1024u) return false;
+        if ((uint32_t)strip_max_resident(threads, lds) >= sh.n_wg + sh.n_tail) return true;
+        if (!measure) return false;
+    }
+    return false;
+}
+
+bool StripRunner::usable(const HbmLattice &h, bool measure)
+{
+    StripShape sh;
+    int threads = 0;
+    return plan(h, measure, sh, threads);
+}
+
+hipError_t StripRunner::run(const HbmLattice *d, const HbmLattice &h, uint32_t t0, uint32_t q0, uint32_t n,
+                            bool measure, hipStream_t s, KernelTimer *tm)
+{
+    StripArgs a;
+    int threads = 0;
+    if (!plan(h, measure, a.sh, threads)) return hipErrorInvalidValue;
+    hipError_t e;
+    if (a.sh.ws_words > ws_words_) {
+        // the stream may still run a launch that uses the old workspace
+        if (ws_) {
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+            (void)hipFree(ws_);
+            ws_ = nullptr;
+        }
+        void *p = nullptr;
+        if ((e = hipMalloc(&p, (size_t)a.sh.ws_words * 4u)) != hipSuccess) return e;
+        ws_ = static_cast<uint32_t *>(p);
+        ws_words_ = a.sh.ws_words;
+    }
+    a.lat = d;
+    a.ws = ws_;
+    a.t0 = t0;
+    a.q0 = q0;
+    a.n_sweeps = n;
+    a.strip_sleep = 1;
+    a.tail_sleep = 8;
+    if (const char *v = std::getenv("MCRG_STRIP_SLEEP")) a.strip_sleep = (uint32_t)std::max(1, std::atoi(v));
+    if (const char *v = std::getenv("MCRG_TAIL_SLEEP")) a.tail_sleep = (uint32_t)std::max(1, std::atoi(v));
+    const int ev = tm ? tm->begin(kKindStrip, s) : -1;
+    if ((e = launch_strip(a, threads, s)) != hipSuccess) return e;
+    if (tm) tm->end(ev, s);
+    launched_ = true;
+    return hipSuccess;
+}
+
+hipError_t StripRunner::failed(hipStream_t s, bool &out)
+{
+    out = false;
+    if (!launched_) return hipSuccess;
+    uint32_t err = 0;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(&err, ws_, sizeof(err), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    launched_ = false;
+    out = err != 0u;
+    return hipSuccess;  // the next launch clears the word
+}
+
+}  // namespace mcrg

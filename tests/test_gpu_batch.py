@@ -194,6 +194,31 @@ def test_swendsen_wang_jobs_in_batch(amd, ctx, orc):
             assert np.array_equal(b.records(j), r_ref), job
 
 
+def test_bisection_batch_shape_matches_oracle(amd, ctx, orc):
+    """The K-bisection's batch (configuration C5) at a few sweeps: L = 1024
+    Swendsen-Wang int8 jobs blocked six times (1024 -> 16) beside L = 128
+    bit-packed Metropolis jobs blocked three times (128 -> 16), at four
+    couplings of the starting bracket [0.43, 0.45], in one batch.  Every
+    job's spins and records equal the oracle's."""
+    jobs = []
+    for q in range(4):
+        J = 0.43 + 0.02 * (q + 0.5) / 4
+        jobs.append(dict(L=1024, coupling=J, seed=5000 + q, n_therm=2, n_meas=4, n_levels=6, sweeps_per_bin=2,
+                         sampler=amd.SAMPLER_SWENDSEN_WANG))
+        for r in range(4):
+            jobs.append(dict(L=128, coupling=J, seed=6000 + 16 * q + r, n_therm=3, n_meas=4, n_levels=3,
+                             sweeps_per_bin=2, sampler=amd.SAMPLER_METROPOLIS))
+    print("bisection batch shape", len(jobs), flush=True)
+    with amd.Batch(ctx, jobs) as b:
+        b.run_all()
+        b.sync()
+        for j, job in enumerate(jobs):
+            s_ref, r_ref = orc.run_chain(job["L"], job["coupling"], job["seed"], job["n_therm"], job["n_meas"],
+                                         job["n_levels"], job["sweeps_per_bin"], sampler=job["sampler"])
+            assert np.array_equal(b.spins(j), s_ref), job
+            assert np.array_equal(b.records(j), r_ref), job
+
+
 def test_autotune_keeps_results_and_picks_a_candidate(amd, ctx, orc):
     """The autotuner runs its candidate configurations on the first launches of
     the real jobs; every job still equals the oracle, and it settles on one of

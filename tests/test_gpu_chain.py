@@ -70,6 +70,45 @@ def test_config2_first_100_sweeps(amd, ctx, orc):
             assert np.array_equal(ch.spins(), s), f"diverged by sweep {upto}"
 
 
+def test_config2_first_100_measured_sweeps_five_levels(amd, ctx, orc):
+    """The plain configuration's shape: L = 2048, |K| = 0.4406868, seed 7, five
+    blockings (2048 -> 64).  Its first 100 sweeps all measured, in bins of 25,
+    enqueued as runs of 1, 9 and 90 sweeps: spins and every record bin equal
+    the oracle's bit for bit."""
+    L, J, seed, nl, spb = 2048, 0.4406868, 7, 5, 25
+    _say("L=2048 first 100 measured sweeps, 5 levels")
+    s_ref, rec_ref = orc.run_chain(L, J, seed, 0, 100, nl, spb)
+    with amd.Chain(ctx, L, J, seed=seed, n_levels=nl, sweeps_per_bin=spb, max_bins=4) as ch:
+        for n in (1, 9, 90):
+            ch.sweep(n, measure=True)
+        assert ch.state() == (100, 100)
+        assert np.array_equal(ch.spins(), s_ref)
+        got = ch.records()
+        assert np.array_equal(got, rec_ref), np.argwhere(got != rec_ref)[:10]
+
+
+@pytest.mark.parametrize("L,nl", [(64, 5), (1024, 4), (4096, 6)])
+def test_strip_and_hbm_schedules_agree(amd, ctx, L, nl, monkeypatch):
+    """Chains of 64 <= L <= 8192 run on the strip schedule (strips held in LDS
+    by a cooperative launch); MCRG_STRIP=0 keeps them on the per-half-sweep
+    HBM schedule.  Both give the same spins, counters and records after
+    unmeasured and measured runs of several lengths."""
+    seed, spb = 31 + L, 6
+    _say("strip vs hbm", L, nl)
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("MCRG_STRIP", flag)
+        with amd.Chain(ctx, L, KC, seed=seed, n_levels=nl, sweeps_per_bin=spb, max_bins=4) as ch:
+            ch.sweep(3)
+            ch.sweep(7, measure=True)
+            ch.sweep(2)
+            ch.sweep(13, measure=True)
+            out.append((ch.spins(), ch.records(), ch.state()))
+    assert out[0][2] == out[1][2] == (25, 20)
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("L,nl,J", [(4, 1, KC), (8, 2, KC), (16, 3, 0.42), (32, 4, KC),
                                     (64, 5, 0.4496804), (128, 6, KC), (256, 5, KC)])
 def test_measurement_records_bit_exact(amd, ctx, orc, L, nl, J):

@@ -87,6 +87,37 @@ def test_critical_exponent_file_and_batch_parity(tmp_path):
         assert e[4] == pytest.approx(t["tau_int"], abs=1e-4) and t["tau_int"] > 0
 
 
+@pytest.mark.parametrize("layout", ["auto", "int8"])
+def test_replica_records_sum_to_the_oracle_chains(orc, tmp_path, layout):
+    """calc_critical_exponent with 3 replicas (the reference's MPI ranks, an
+    uneven split_samples share) and 5 bins: each level's lambda in the
+    reference-format file equals the one formed from the per-bin sum of the
+    three oracle chains with the same seeds, on the batched (auto) and on the
+    int8 chain layout."""
+    from mcrg_amd.workloads import splitmix64
+    N, n_eq, n, R, bins, seed = 16, 50, 301, 3, 5, 9
+    K = -0.4406868
+    _run([RUN, "--N", str(N), "--eq", str(n_eq), "--samples", str(n), "--K", repr(K), "--replicas", str(R),
+          "--bins", str(bins), "--seed", str(seed), "--layout", layout, "--outdir", str(tmp_path), "--quiet"],
+         tmp_path)
+    rows = _read_rows(tmp_path / "critical_exponent_N_16_K_-0.4406868.txt")
+    nl = 3
+    loc = [orc.split_samples(r, R, n) for r in range(R)]
+    assert loc == [99, 101, 101]
+    spb = -(-max(loc) // bins)
+    n_bins = -(-max(loc) // spb)
+    total = np.zeros((n_bins, orc.record_size(nl)), dtype=np.int64)
+    for r in range(R):
+        _, rec = orc.run_chain(N, -K, splitmix64(seed + r), n_eq, loc[r], nl, spb)
+        total[:rec.shape[0]] += rec
+    tot = total.sum(axis=0)
+    assert len(rows) == nl
+    for lvl, row in enumerate(rows):
+        lam = orc.leading_eigenvalue(orc.rg_matrix(tot, nl, lvl, 2))
+        assert row[1] == pytest.approx(lam, abs=2e-10), (lvl, row, lam)
+        assert row[2] == pytest.approx(math.log(2) / math.log(lam), abs=2e-10)
+
+
 def test_replicas_and_hbm_schedule(tmp_path):
     # N = 4 runs on the HBM schedule (below the LDS-resident minimum), 3 replicas
     _run([RUN, "--N", "4", "--eq", "50", "--samples", "300", "--K", "-0.3", "--replicas", "3",
