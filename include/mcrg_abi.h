@@ -135,6 +135,20 @@ mcrg_status mcrg_chain_set_coupling(mcrg_chain ch, double coupling);
  * calc_critical_exponent (src/mcrg.cpp:72-98). */
 mcrg_status mcrg_chain_sweep(mcrg_chain ch, int32_t n_sweeps, int32_t measure);
 
+/* Allocation-free variant of mcrg_chain_sweep.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It runs the sweeps on the strip schedule (the Metropolis sampler,
+ * 64 <= L <= 8192, and L = 16384 for unmeasured sweeps) with the launch's
+ * workspace in a static device array of 4 MiB per device, located once per
+ * device when a context is created, and calls no allocating routine; other
+ * chains are rejected with MCRG_ERR_INVALID_ARG.  Spins, counters and records
+ * are bit-identical to mcrg_chain_sweep's.  Every call on a device shares that
+ * device's array, which keeps the last call's contents (each launch clears
+ * what it uses before it starts): calls that overlap on one device race, so
+ * enqueue them on one chain's stream or synchronise between them. */
+mcrg_status mcrg_chain_sweep_static(mcrg_chain ch, int32_t n_sweeps, int32_t measure);
+
 /* Waits for the chain's queued work. */
 mcrg_status mcrg_chain_sync(mcrg_chain ch);
 

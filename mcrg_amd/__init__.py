@@ -113,6 +113,7 @@ _SIGS = {
     "mcrg_chain_destroy": (_c.c_int32, [_p]),
     "mcrg_chain_set_coupling": (_c.c_int32, [_p, _c.c_double]),
     "mcrg_chain_sweep": (_c.c_int32, [_p, _c.c_int32, _c.c_int32]),
+    "mcrg_chain_sweep_static": (_c.c_int32, [_p, _c.c_int32, _c.c_int32]),
     "mcrg_chain_sync": (_c.c_int32, [_p]),
     "mcrg_chain_get_spins": (_c.c_int32, [_p, _p, _c.c_int64]),
     "mcrg_chain_set_spins": (_c.c_int32, [_p, _p, _c.c_int64]),
@@ -298,6 +299,10 @@ class Chain:
 
     def sweep(self, n, measure=False):
         check(_lib.mcrg_chain_sweep(self.handle, n, 1 if measure else 0))
+
+    def sweep_static(self, n, measure=False):
+        """mcrg_chain_sweep_static: the same sweeps without allocating."""
+        check(_lib.mcrg_chain_sweep_static(self.handle, n, 1 if measure else 0))
 
     def sync(self):
         check(_lib.mcrg_chain_sync(self.handle))

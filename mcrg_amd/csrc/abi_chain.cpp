@@ -161,6 +161,9 @@ mcrg_status mcrg_ctx_create(int32_t device, mcrg_ctx *out)
         return MCRG_ERR_OUT_OF_BUDGET;
     }
     c->device = device;
+    // the allocation-free sweep's workspace on this device (see strip.hpp); a
+    // failure is reported by that entry point, not here
+    (void)static_scratch_setup(device);
     *out = c;
     return MCRG_OK;
 }
@@ -311,6 +314,38 @@ mcrg_status mcrg_chain_sweep(mcrg_chain ch, int32_t n_sweeps, int32_t measure)
     else
         MCRG_HIP(ch->runner.run(ch->d_desc, 1, d.L, d.n_levels, d.sampler, (uint32_t)n_sweeps, measure != 0,
                                 ch->stream, tm));
+    ch->t += (uint32_t)n_sweeps;
+    if (measure) ch->n_measured += (uint32_t)n_sweeps;
+    return MCRG_OK;
+}
+
+mcrg_status mcrg_chain_sweep_static(mcrg_chain ch, int32_t n_sweeps, int32_t measure)
+{
+    MCRG_REQUIRE(ch != nullptr, "chain is null");
+    MCRG_REQUIRE(n_sweeps >= 0, "n_sweeps must be >= 0");
+    if (n_sweeps == 0) return MCRG_OK;
+    const mcrg_chain_desc &d = ch->desc;
+    if (measure) {
+        const uint64_t last = (uint64_t)ch->n_measured + (uint64_t)n_sweeps - 1;
+        if (last / (uint64_t)d.sweeps_per_bin >= (uint64_t)d.max_bins) {
+            set_error("record bins exhausted: raise max_bins or sweeps_per_bin");
+            return MCRG_ERR_CAPACITY;
+        }
+    }
+    MCRG_REQUIRE((uint64_t)ch->t + (uint64_t)n_sweeps <= 0xFFFFFFFFull, "sweep counter overflow");
+    MCRG_HIP(hipSetDevice(ch->ctx->device));
+    uint32_t *ws = static_scratch(ch->ctx->device);
+    if (!ws) {
+        set_error("the static workspace of this device could not be located (hipGetSymbolAddress failed at context "
+                  "creation)");
+        return MCRG_ERR_DEVICE;
+    }
+    const size_t need = ch->strip.workspace_words(ch->host, measure != 0);
+    MCRG_REQUIRE(need > 0, "mcrg_chain_sweep_static runs chains on the strip schedule only: the Metropolis sampler, "
+                           "64 <= L <= 8192 (L = 16384 for unmeasured sweeps)");
+    MCRG_REQUIRE(need <= kStaticScratchWords, "this chain needs more workspace than the static array holds");
+    MCRG_HIP(ch->strip.run(ch->d_desc, ch->host, ch->t, ch->n_measured, (uint32_t)n_sweeps, measure != 0,
+                           ch->stream, nullptr, ws, kStaticScratchWords));
     ch->t += (uint32_t)n_sweeps;
     if (measure) ch->n_measured += (uint32_t)n_sweeps;
     return MCRG_OK;

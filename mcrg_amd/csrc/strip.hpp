@@ -149,6 +149,17 @@ namespace mcrg {
 
 struct KernelTimer;
 
+// The allocation-free strip launches share one device array per device of
+// kStaticScratchWords 32-bit words (4 MiB): enough for every shape the strip
+// schedule runs (at most about 2.1 MiB, L = 8192 measured and L = 16384).
+constexpr size_t kStaticScratchWords = size_t(1) << 20;
+constexpr int kMaxDevices = 64;
+// Looks up the array's address on `device` (the current device), once per
+// device; later calls return the first answer.
+hipError_t static_scratch_setup(int device);
+// Its address on `device`, or null when setup failed or has not run.
+uint32_t *static_scratch(int device);
+
 // Runs a chain's sweeps on the strip schedule when its shape allows it: the
 // Metropolis sampler, 64 <= L <= 8192 (L = 16384 for unmeasured sweeps, whose
 // strips alone fill the LDS), and every workgroup of the launch resident at
@@ -161,8 +172,12 @@ class StripRunner {
     StripRunner &operator=(const StripRunner &) = delete;
     ~StripRunner();
     bool usable(const HbmLattice &h, bool measure);
+    // Workspace words a launch of this shape needs (0 when it cannot run here).
+    size_t workspace_words(const HbmLattice &h, bool measure);
+    // With fixed_ws the launch uses that workspace of fixed_words words and
+    // allocates nothing (hipErrorInvalidValue when it needs more).
     hipError_t run(const HbmLattice *d, const HbmLattice &h, uint32_t t0, uint32_t q0, uint32_t n, bool measure,
-                   hipStream_t s, KernelTimer *tm);
+                   hipStream_t s, KernelTimer *tm, uint32_t *fixed_ws = nullptr, size_t fixed_words = 0);
     // After the stream has drained: true when a launch since the last call
     // gave up on a wait (its error word is set, then cleared here).
     hipError_t failed(hipStream_t s, bool &out);
@@ -171,6 +186,7 @@ class StripRunner {
     bool plan(const HbmLattice &h, bool measure, StripShape &sh, int &threads);
     uint32_t *ws_ = nullptr;
     size_t ws_words_ = 0;
+    uint32_t *last_ws_ = nullptr;  // workspace of the last launch (its error word)
     bool launched_ = false;
     int cus_ = -1;
 };

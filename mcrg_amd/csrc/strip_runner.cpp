@@ -2,11 +2,46 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "runtime.hpp"
 #include "strip.hpp"
 
 namespace mcrg {
+
+// Workspace of the allocation-free strip launches (mcrg_chain_sweep_static):
+// each device the library runs on holds this array until the process ends.
+// In host code its name is only a stand-in for the device copy: the launches
+// use the per-device address that hipGetSymbolAddress returns.
+static __device__ __align__(256) uint32_t scratch[kStaticScratchWords];
+
+namespace {
+std::mutex g_scratch_mu;
+uint32_t *g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+}  // namespace
+
+hipError_t static_scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        // the calling thread has made `device` current
+        void *p = nullptr;
+        g_scratch_err[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_ptr[device] = g_scratch_err[device] == hipSuccess ? static_cast<uint32_t *>(p) : nullptr;
+        g_scratch_done[device] = true;
+    }
+    return g_scratch_err[device];
+}
+
+uint32_t *static_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
 
 StripRunner::~StripRunner()
 {
@@ -67,13 +102,16 @@ bool StripRunner::usable(const HbmLattice &h, bool measure)
 }
 
 hipError_t StripRunner::run(const HbmLattice *d, const HbmLattice &h, uint32_t t0, uint32_t q0, uint32_t n,
-                            bool measure, hipStream_t s, KernelTimer *tm)
+                            bool measure, hipStream_t s, KernelTimer *tm, uint32_t *fixed_ws, size_t fixed_words)
 {
     StripArgs a;
     int threads = 0;
     if (!plan(h, measure, a.sh, threads)) return hipErrorInvalidValue;
     hipError_t e;
-    if (a.sh.ws_words > ws_words_) {
+    if (fixed_ws) {
+        if (a.sh.ws_words > fixed_words) return hipErrorInvalidValue;
+        a.ws = fixed_ws;
+    } else if (a.sh.ws_words > ws_words_) {
         // the stream may still run a launch that uses the old workspace
         if (ws_) {
             if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
@@ -85,8 +123,8 @@ hipError_t StripRunner::run(const HbmLattice *d, const HbmLattice &h, uint32_t t
         ws_ = static_cast<uint32_t *>(p);
         ws_words_ = a.sh.ws_words;
     }
+    if (!fixed_ws) a.ws = ws_;
     a.lat = d;
-    a.ws = ws_;
     a.t0 = t0;
     a.q0 = q0;
     a.n_sweeps = n;
@@ -98,7 +136,15 @@ hipError_t StripRunner::run(const HbmLattice *d, const HbmLattice &h, uint32_t t
     if ((e = launch_strip(a, threads, s)) != hipSuccess) return e;
     if (tm) tm->end(ev, s);
     launched_ = true;
+    last_ws_ = a.ws;
     return hipSuccess;
+}
+
+size_t StripRunner::workspace_words(const HbmLattice &h, bool measure)
+{
+    StripShape sh;
+    int threads = 0;
+    return plan(h, measure, sh, threads) ? (size_t)sh.ws_words : 0u;
 }
 
 hipError_t StripRunner::failed(hipStream_t s, bool &out)
@@ -107,7 +153,7 @@ hipError_t StripRunner::failed(hipStream_t s, bool &out)
     if (!launched_) return hipSuccess;
     uint32_t err = 0;
     hipError_t e;
-    if ((e = hipMemcpyAsync(&err, ws_, sizeof(err), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(&err, last_ws_, sizeof(err), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
     launched_ = false;
     out = err != 0u;
