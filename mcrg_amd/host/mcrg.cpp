@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <iomanip>
+#include <limits>
 #include <mutex>
 #include <sstream>
 
@@ -439,7 +440,153 @@ double crossing(const std::vector<double> &J, const std::vector<double> &D, cons
     const double b = (S * Sxy - Sx * Sy) / det, a = (Sy - b * Sx) / S;
     return b != 0.0 ? -a / b : std::nan("");
 }
+
 }  // namespace
+
+namespace mcrg_cpp {
+namespace {
+// Weighted least-squares polynomial of degree deg <= 3 in x = (J - mid) / span
+// through the points idx; false when singular.  chi2 = sum w (D - fit)^2.
+bool poly_fit(const std::vector<double> &J, const std::vector<double> &D, const std::vector<double> &w,
+              const std::vector<size_t> &idx, int deg, double mid, double span, double cf[4], double &chi2)
+{
+    double M[4][5] = {};
+    for (size_t i : idx) {
+        const double x = (J[i] - mid) / span;
+        const double p[4] = {1.0, x, x * x, x * x * x};
+        for (int r = 0; r <= deg; ++r) {
+            for (int c = 0; c <= deg; ++c) M[r][c] += w[i] * p[r] * p[c];
+            M[r][deg + 1] += w[i] * p[r] * D[i];
+        }
+    }
+    for (int c = 0; c <= deg; ++c) {  // Gauss-Jordan with partial pivoting
+        int piv = c;
+        for (int r = c + 1; r <= deg; ++r)
+            if (std::fabs(M[r][c]) > std::fabs(M[piv][c])) piv = r;
+        if (M[piv][c] == 0.0) return false;
+        for (int j = 0; j <= deg + 1; ++j) std::swap(M[c][j], M[piv][j]);
+        for (int r = 0; r <= deg; ++r) {
+            if (r == c) continue;
+            const double f = M[r][c] / M[c][c];
+            for (int j = 0; j <= deg + 1; ++j) M[r][j] -= f * M[c][j];
+        }
+    }
+    for (int r = 0; r < 4; ++r) cf[r] = r <= deg ? M[r][deg + 1] / M[r][r] : 0.0;
+    chi2 = 0.0;
+    for (size_t i : idx) {
+        const double x = (J[i] - mid) / span, e = D[i] - (cf[0] + x * (cf[1] + x * (cf[2] + x * cf[3])));
+        chi2 += w[i] * e * e;
+    }
+    return std::isfinite(chi2);
+}
+
+// Zero of the polynomial cf in J over [lo, hi] nearest mid (NaN when none).
+double poly_zero(const double cf[4], double mid, double span, double lo, double hi)
+{
+    auto f = [&](double J_) {
+        const double x = (J_ - mid) / span;
+        return cf[0] + x * (cf[1] + x * (cf[2] + x * cf[3]));
+    };
+    constexpr int kScan = 512;
+    double root = std::nan(""), dist = std::numeric_limits<double>::infinity();
+    double pa = lo, fa = f(pa);
+    for (int i = 1; i <= kScan; ++i) {
+        const double pb = lo + (hi - lo) * i / kScan, fb = f(pb);
+        if (fa == 0.0 || fa * fb < 0.0) {
+            double a0 = pa, b0 = pb, f0 = fa;
+            for (int it = 0; it < 100 && f0 != 0.0 && b0 - a0 > 0.0; ++it) {  // bisection
+                const double m = 0.5 * (a0 + b0), fm = f(m);
+                if (m <= a0 || m >= b0) break;
+                if ((fm < 0.0) == (f0 < 0.0)) {
+                    a0 = m;
+                    f0 = fm;
+                } else {
+                    b0 = m;
+                }
+            }
+            const double r = f0 == 0.0 ? a0 : 0.5 * (a0 + b0);
+            if (std::fabs(r - mid) < dist) {
+                dist = std::fabs(r - mid);
+                root = r;
+            }
+        }
+        pa = pb;
+        fa = fb;
+    }
+    if (fa == 0.0 && std::fabs(hi - mid) < dist) root = hi;
+    return root;
+}
+}  // namespace
+
+double zero_crossing(const std::vector<double> &J_in, const std::vector<double> &D_in,
+                     const std::vector<double> &w_in)
+{
+    // points with a positive weight
+    std::vector<double> J, D, w;
+    for (size_t i = 0; i < std::min(J_in.size(), std::min(D_in.size(), w_in.size())); ++i)
+        if (w_in[i] > 0.0 && std::isfinite(w_in[i]) && std::isfinite(D_in[i]) && std::isfinite(J_in[i])) {
+            J.push_back(J_in[i]);
+            D.push_back(D_in[i]);
+            w.push_back(w_in[i]);
+        }
+    const size_t n = J.size();
+    // the sign change between neighbours (in J) whose signs are the most
+    // significant, |D| sqrt(w) standard errors
+    std::vector<size_t> byJ(n);
+    for (size_t i = 0; i < n; ++i) byJ[i] = i;
+    std::sort(byJ.begin(), byJ.end(), [&](size_t x, size_t y) { return J[x] < J[y]; });
+    double best = -1.0, mid = std::nan("");
+    for (size_t k = 0; k + 1 < n; ++k) {
+        const size_t i = byJ[k], j = byJ[k + 1];
+        if (!(D[i] * D[j] <= 0.0) || D[i] == D[j]) continue;
+        const double s = std::min(std::fabs(D[i]) * std::sqrt(w[i]), std::fabs(D[j]) * std::sqrt(w[j]));
+        if (s > best) {
+            best = s;
+            mid = J[i] + (J[j] - J[i]) * D[i] / (D[i] - D[j]);
+        }
+    }
+    if (!std::isfinite(mid)) return crossing(J, D, w);
+    std::vector<size_t> order(byJ);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return std::fabs(J[x] - mid) < std::fabs(J[y] - mid); });
+    // the widest window of nearest points whose weighted quadratic fits
+    // (chi2 within 3 standard deviations of its mean) and crosses zero there
+    for (size_t k = n; k >= 4; --k) {
+        std::vector<size_t> idx(order.begin(), order.begin() + (long)k);
+        double lo = J[idx[0]], hi = lo;
+        for (size_t i : idx) {
+            lo = std::min(lo, J[i]);
+            hi = std::max(hi, J[i]);
+        }
+        double cf[4], chi2;
+        if (!(hi > lo) || !poly_fit(J, D, w, idx, 2, mid, hi - lo, cf, chi2)) continue;
+        const double dof = (double)k - 3.0;
+        if (chi2 > dof + 3.0 * std::sqrt(2.0 * dof)) continue;
+        const double r = poly_zero(cf, mid, hi - lo, lo, hi);
+        if (std::isfinite(r)) return r;
+    }
+    // none: the cubic through the four nearest points, else their straight line
+    std::vector<size_t> idx(order.begin(), order.begin() + (long)std::min<size_t>(n, 4));
+    std::vector<double> Jk, Dk, wk;
+    double lo = mid, hi = mid;
+    for (size_t i : idx) {
+        Jk.push_back(J[i]);
+        Dk.push_back(D[i]);
+        wk.push_back(w[i]);
+        lo = std::min(lo, J[i]);
+        hi = std::max(hi, J[i]);
+    }
+    double cf[4], chi2;
+    if (idx.size() == 4 && hi > lo && poly_fit(J, D, w, idx, 3, mid, hi - lo, cf, chi2)) {
+        const double r = poly_zero(cf, mid, hi - lo, lo, hi);
+        if (std::isfinite(r)) return r;
+    }
+    const double line = crossing(Jk, Dk, wk);
+    return std::isfinite(line) ? line : mid;
+}
+}  // namespace mcrg_cpp
+
+using mcrg_cpp::zero_crossing;
 
 MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect_critical_point(
     int n_rounds, int n_k, double J_lo, double J_hi, int n_large, int L_large, int n_small, int L_small,
@@ -534,17 +681,17 @@ MonteCarloRenormalizationGroup::Bisection MonteCarloRenormalizationGroup::bisect
             for (int m = 0; m < n_match; ++m) br.D_level.push_back(D_of(q, m, -1));
             w[(size_t)q] = var > 0.0 ? 1.0 / var : 0.0;
         }
-        br.Jc = crossing(br.J, br.D, w);
+        br.Jc = zero_crossing(br.J, br.D, w);
         for (int m = 0; m < n_match; ++m) {
             std::vector<double> Dm((size_t)n_k);
             for (int q = 0; q < n_k; ++q) Dm[(size_t)q] = br.D_level[(size_t)q * n_match + m];
-            br.Jc_level.push_back(crossing(br.J, Dm, w));
+            br.Jc_level.push_back(zero_crossing(br.J, Dm, w));
         }
         double var = 0.0;
         std::vector<double> Dq((size_t)n_k);
         for (int g = 0; g < G; ++g) {
             for (int q = 0; q < n_k; ++q) Dq[(size_t)q] = D_of(q, deep, g);
-            const double c = crossing(br.J, Dq, w);
+            const double c = zero_crossing(br.J, Dq, w);
             var += (c - br.Jc) * (c - br.Jc);
         }
         br.Jc_err = std::sqrt(jk * var);

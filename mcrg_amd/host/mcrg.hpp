@@ -81,6 +81,17 @@ uint64_t next_seed();
 std::string rounded_str(double num, int precision);  // get_rounded_str, src/definitions.cpp:70-77
 bool write_iter(int i);                                // src/definitions.cpp:46-68
 int split_samples(int rank, int n_processes, int n_samples);  // src/definitions.cpp:79-87
+// Zero of D(J) sampled at couplings J with weights w (1 / variance), as the
+// K-bisection locates it.  Over a wide bracket D(J) bends (it saturates on
+// either side of the critical coupling), so a straight line through every
+// point is biased.  Instead, among the points with positive weight: the sign
+// change between neighbours whose signs are the most significant (|D| sqrt(w)),
+// then the widest window of points nearest it whose weighted quadratic fits
+// (chi^2 within three standard deviations of its mean) and crosses zero inside
+// the window, that zero nearest the sign change; failing that, the cubic
+// through the four nearest points.  Without a sign change: the weighted
+// straight line through every point, an extrapolation.  NaN when undefined.
+double zero_crossing(const std::vector<double> &J, const std::vector<double> &D, const std::vector<double> &w);
 }  // namespace mcrg_cpp
 
 class Lattice {
@@ -170,9 +181,10 @@ class MonteCarloRenormalizationGroup {
     // correlation per site of L_large blocked k + m times with L_small blocked
     // m times (k = log2(L_large / L_small); matched levels m = 0..3 while
     // L_small / 2^m >= 16: same final size).  D(J) = large - small at the
-    // deepest matched level changes sign at the critical coupling; a weighted
-    // straight line through D(J) gives the crossing J0 and its jackknife error
-    // over groups of independent chains, and the next bracket is
+    // deepest matched level changes sign at the critical coupling; a local fit
+    // around the sign change (mcrg_cpp::zero_crossing) gives the crossing J0
+    // and its jackknife error over groups of independent chains, and the next
+    // bracket is
     // J0 +- max(3 err, width / 8) (a crossing outside the bracket moves it by at
     // most one width).  J = |K|.
     struct BisectionRound {

@@ -152,7 +152,8 @@ def test_cli_rejects_bad_coupling(tmp_path):
 def test_bisection_driver(tmp_path):
     """K-bisection with two lattice sizes in one batch (L = 64 on the LDS-resident
     schedule, L = 16 beside it): a crossing inside the first bracket, each
-    round's bracket centred on the previous crossing."""
+    round's bracket centred on the previous crossing, every round's crossing
+    consistent with the last."""
     out = _run([RUN, "--bisect", "2", "--n-k", "8", "--n-large", "16", "--L-large", "64", "--n-small", "64",
                 "--L-small", "16", "--eq", "500", "--samples", "4000", "--bins", "20", "--quiet"], tmp_path)
     r = json.loads(out.strip().splitlines()[-1])
@@ -163,6 +164,10 @@ def test_bisection_driver(tmp_path):
     assert (lo2 + hi2) / 2 == pytest.approx(jc1, abs=1e-9)
     assert r["Jc"] == jc2 and r["Jc_err"] == e2
     assert abs(r["Jc"] - r["exact"]) < 0.01
+    # no round's crossing is biased by the width of its bracket: each lies
+    # within 5 standard errors of the final one
+    for lo, hi, jc, e in r["round_Jc"]:
+        assert abs(jc - r["Jc"]) < 5.0 * max(e, r["Jc_err"])
     assert r["site_updates"] == 2 * (16 * 64 * 64 + 64 * 16 * 16) * 4500
 
 
