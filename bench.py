@@ -82,6 +82,9 @@ MEASURED_VALU = {
 }
 
 
+KIND_STRIP = 4  # mcrg_ctx_kernel_time kind of the strip kernel
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -187,23 +190,39 @@ def matched_yt(mcrg_amd, ctx, jobs, sweeps, plain_recs):
     return {"jobs": len(jobs), "sweeps": sweeps, "records_identical": same}
 
 
-def run_plain(mcrg_amd, ctx, L, nl, steps, measure=True, warm=36):
+def run_plain(mcrg_amd, ctx, L, nl, steps, measure=True, warm=36, kernels=None):
     """One plain int8 chain (|K| = 0.4406868, seed 7): `warm` sweeps (which
-    also capture the replayed graph), then `steps` timed sweeps."""
-    spb = max(1, min(steps + warm, mcrg_amd.max_sweeps_per_bin(L)))  # int64-safe bins (L = 16384: 31 sweeps)
+    also capture the replayed graph), then `steps` timed sweeps.  When
+    `kernels` is a dict it receives the strip kernel's launches and duration
+    over the timed sweeps (HIP events on the chain's stream; none when the
+    chain runs on the per-half-sweep schedule)."""
+    total = warm + 2 * steps  # the timed sweeps run twice: wall clock, then HIP events
+    spb = max(1, min(total, mcrg_amd.max_sweeps_per_bin(L)))  # int64-safe bins (L = 16384: 31 sweeps)
     with mcrg_amd.Chain(ctx, L, 0.4406868, 7, n_levels=nl, sweeps_per_bin=spb,
-                        max_bins=-(-(steps + warm) // spb)) as ch:
+                        max_bins=-(-total // spb)) as ch:
         ch.sweep(warm, measure=measure)
         ch.sync()
+        n0, ms0 = ctx.kernel_time(KIND_STRIP)
         t0 = time.perf_counter()
         ch.sweep(steps, measure=measure)
         ch.sync()
         dt = time.perf_counter() - t0
+        # the same sweeps once more with the kernel timed (events cost a little)
+        ctx.set_timing(True)
+        ch.sweep(steps, measure=measure)
+        ch.sync()
+        ctx.set_timing(False)
+        n1, ms1 = ctx.kernel_time(KIND_STRIP)
+    if kernels is not None and n1 > n0:
+        kernels.update({"kernel": "k_strip", "launches": n1 - n0, "sweeps_per_launch": steps,
+                        "avg_ms": (ms1 - ms0) / (n1 - n0),
+                        "updates_per_ns_kernel": L * L * steps * (n1 - n0) / ((ms1 - ms0) * 1e6),
+                        "source": "HIP events on the chain's stream"})
     return L * L * steps / (dt * 1e9)
 
 
-def run_plain_2048(mcrg_amd, ctx, steps):
-    return run_plain(mcrg_amd, ctx, 2048, 5, steps, True, 20)
+def run_plain_2048(mcrg_amd, ctx, steps, kernels=None):
+    return run_plain(mcrg_amd, ctx, 2048, 5, steps, True, 20, kernels)
 
 
 def run_serving(mcrg_amd, workloads, ctx, rate, duration, drain_limit, quantum, threads):
@@ -405,6 +424,7 @@ def main():
     ap.add_argument("--min-team", type=int, default=16, help="--only batched: lanes per lattice at least")
     ap.add_argument("--max-slot-lds", type=int, default=0, help="--only batched: LDS cap per slot (0 = default)")
     ap.add_argument("--threads-large", type=int, default=0, help="--only batched: workgroup size for L = 512")
+    ap.add_argument("--max-team", type=int, default=0, help="--only batched: lanes per lattice at most (0 = threads)")
     ap.add_argument("--sub-chunks", type=int, default=0,
                     help="parts per slot and launch (0 = automatic; the autotuned run uses 0)")
     ap.add_argument("--plain-L", type=int, default=16384, help="--only plain: lattice side")
@@ -438,14 +458,19 @@ def main():
     ctx = mcrg_amd.Context(0)
     if args.only == "batched":
         cfg = dict(min_team=args.min_team, max_slot_lds=args.max_slot_lds, threads_large=args.threads_large,
-                   sub_chunks=args.sub_chunks)
+                   sub_chunks=args.sub_chunks, max_team=args.max_team)
         value, dt, _, _ = run_batched(mcrg_amd, ctx, jobs, args.steps, args.warmup, args.sweeps_per_step,
                                       args.threads, cfg=cfg)
         log(f"bench: batched {value:.1f} updates/ns ({dt:.3f} s timed), {cfg}")
         ctx.close()
         return 0
     if args.only == "plain":
-        rate = run_plain(mcrg_amd, ctx, args.plain_L, args.plain_levels, args.steps, bool(args.plain_measure))
+        kern = {}
+        rate = run_plain(mcrg_amd, ctx, args.plain_L, args.plain_levels, args.steps, bool(args.plain_measure),
+                         kernels=kern)
+        if kern:
+            log(f"bench: {kern['kernel']} {kern['avg_ms']:.4f} ms per launch of {kern['sweeps_per_launch']} sweeps "
+                f"({kern['updates_per_ns_kernel']:.1f} updates/ns inside the kernel, {kern['source']})")
         log(f"bench: plain L={args.plain_L} levels={args.plain_levels} measure={args.plain_measure}: "
             f"{rate:.1f} updates/ns, {args.plain_L ** 2 / rate / 1e3:.1f} us per sweep")
         ctx.close()
@@ -480,6 +505,7 @@ def main():
     log(f"bench: resident kernel {avg_ms:.3f} ms/launch over {n_launch} launches -> {achieved:.1f} GB/s algorithmic")
 
     plain_mixed = plain_2048 = matched = None
+    plain_2048_kernel = {}
     if not args.no_plain:
         sample = jobs[:args.plain_sample]
         plain_mixed, pdt, plain_recs = run_plain_mixed(mcrg_amd, ctx, sample, args.plain_sweeps)
@@ -487,7 +513,7 @@ def main():
             f"({pdt:.3f} s timed)")
         matched = matched_yt(mcrg_amd, ctx, sample, 2 * args.plain_sweeps, plain_recs)
         log(f"bench: batched records of the same replicas identical: {matched['records_identical']}")
-        plain_2048 = run_plain_2048(mcrg_amd, ctx, 500)
+        plain_2048 = run_plain_2048(mcrg_amd, ctx, 500, plain_2048_kernel)
         log(f"bench: plain L=2048: {plain_2048:.1f} updates/ns")
     valu = valu_roofline(cfg_key, sites, args.sweeps_per_step, avg_ms, dev["clock_khz"], dev["compute_units"])
     serving = []
@@ -538,6 +564,7 @@ def main():
                                        "sweep, bit-packed streaming model)"}),
         "plain_mixed": plain_mixed,
         "plain_l2048": plain_2048,
+        "plain_l2048_kernel": plain_2048_kernel or None,
         "batched_over_plain_mixed": (value / plain_mixed) if plain_mixed else None,
         "matched_yt_error": matched,
         "serving": serving,

@@ -399,7 +399,8 @@ mcrg_status mcrg_critical_point_step_bins(const int64_t *rec_large, int32_t n_bi
 /* Kernel-duration statistics measured with HIP events on the stream that
  * launched the kernel, enabled per context.  kind: 0 = half-sweep kernel,
  * 1 = level-0 measurement kernel, 2 = levels/accumulate kernel,
- * 3 = batched resident kernel. */
+ * 3 = batched resident kernel, 4 = strip kernel (one launch per
+ * mcrg_chain_sweep call on the strip schedule). */
 mcrg_status mcrg_ctx_set_timing(mcrg_ctx ctx, int32_t enable);
 mcrg_status mcrg_ctx_kernel_time(mcrg_ctx ctx, int32_t kind, int64_t *launches, double *total_ms);
 
