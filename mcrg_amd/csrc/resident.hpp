@@ -80,6 +80,32 @@ __host__ __device__ inline SlotLayout slot_layout(uint32_t L, uint32_t G, uint32
     return s;
 }
 
+// First blocked level whose rows fit one per lane of a team of T lanes (at
+// most one wave), or 0 when no level from 1 to nl does: the levels from it on
+// are measured in registers (the "pyramid").
+__host__ __device__ inline uint32_t pyramid_level(uint32_t L, uint32_t T, uint32_t nl)
+{
+    const uint32_t lanes = T < 64u ? T : 64u;
+    for (uint32_t l = 1; l <= nl; ++l)
+        if ((L >> l) <= lanes) return l;
+    return 0u;
+}
+
+// Slot bodies the resident kernel is compiled with: the wide body for teams of
+// at least a wave on L >= 64, whose register pyramid starts at rows of 64
+// (L >= 128) or 32 sites (L = 64), unrolled for those two shapes and without
+// the small-lattice code paths; the generic body for everything else.
+enum SlotShape : uint32_t { kShapeGeneric = 0, kShapeRows64 = 1, kShapeRows32 = 2 };
+
+__host__ __device__ inline uint32_t slot_shape(uint32_t L, uint32_t G, uint32_t nl, uint32_t NT)
+{
+    const uint32_t T = NT / G;
+    const uint32_t pl = pyramid_level(L, T, nl);
+    if (pl == 0u || T < 64u) return kShapeGeneric;
+    const uint32_t Lp = L >> pl;
+    return Lp == 64u ? kShapeRows64 : Lp == 32u && L >= 64u ? kShapeRows32 : kShapeGeneric;
+}
+
 // Admits n lattices: lats[idx[k]] = stage[k], its records zeroed, its spins
 // drawn (stage and idx in device memory).
 hipError_t launch_packed_init(PackedLattice *lats, const PackedLattice *stage, const int32_t *idx, int n,

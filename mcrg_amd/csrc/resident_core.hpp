@@ -201,16 +201,6 @@ __device__ __forceinline__ void sweep_colour(const uint32_t *meta, uint32_t *spi
     }
 }
 
-// First level whose rows fit one per lane of the team's first wave, or 0 when
-// no level from 1 to nl does (L = 8, or L = 16 with one level).
-__device__ __forceinline__ uint32_t pyramid_level(uint32_t L, uint32_t T, uint32_t nl)
-{
-    const uint32_t lanes = T < 64u ? T : 64u;
-    for (uint32_t l = 1; l <= nl; ++l)
-        if ((L >> l) <= lanes) return l;
-    return 0u;
-}
-
 // Row y+1 (mod Ll) for lane y.  A team of a whole wave rotates the wave by one
 // lane with DPP (no LDS round trip) and patches the wrap-around row; smaller
 // teams shuffle within their segment.
@@ -302,12 +292,93 @@ __device__ __forceinline__ void pyramid(const uint32_t *lg, const uint32_t *cg, 
     }
 }
 
+// pyramid() for a starting row width LP and shuffle segment WIDTH known at
+// compile time: its levels are unrolled, so every width test (the group sums,
+// the row rotation, the 64-site rows) folds away.  The same operations in the
+// same order, so the counts and blocked rows are identical.
+template <uint32_t LP, uint32_t WIDTH>
+__device__ __forceinline__ void pyramid_t(const uint32_t *lg, const uint32_t *cg, uint32_t *cntg, uint32_t L,
+                                          uint32_t pl, uint32_t nl, uint32_t lvl_off, uint32_t coin_off,
+                                          uint32_t y)
+{
+    uint32_t lo = 0, hi = 0;
+    if (y < LP) {
+        if (LP == 64u) {
+            lo = lg[lvl_off + 2u * y];
+            hi = lg[lvl_off + 2u * y + 1u];
+        } else {
+            lo = rm_row_small(lg + lvl_off, LP, y);
+        }
+    }
+    uint32_t coff = coin_off;
+#pragma unroll
+    for (uint32_t j = 0; (LP >> j) >= 2u; ++j) {
+        const uint32_t l = pl + j;
+        if (l > nl) break;
+        const uint32_t Ll = LP >> j;
+        const bool wide = Ll == 64u;
+        const uint32_t Lb = Ll >> 1;
+        uint32_t coin = 0;
+        if (l < nl && y < Ll && (y & 1u) == 0u) {
+            const uint32_t bm = (y >> 1) * Lb;
+            coin = cg[coff + (bm >> 5)] >> (bm & 31u);
+        }
+        const uint32_t dlo = row_next(lo, y, Ll, WIDTH);
+        const uint32_t dhi = wide ? row_next(hi, y, Ll, WIDTH) : 0u;
+        uint32_t nn_nnn = 0, dp = 0;
+        if (y < Ll) {
+            if (wide) {
+                const uint32_t rlo = (lo >> 1) | (hi << 31), rhi = (hi >> 1) | (lo << 31);
+                const uint32_t glo = (dlo >> 1) | (dhi << 31), ghi = (dhi >> 1) | (dlo << 31);
+                const uint32_t dnn = (uint32_t)(popc(lo ^ rlo) + popc(hi ^ rhi) + popc(lo ^ dlo) + popc(hi ^ dhi));
+                const uint32_t dnnn =
+                    (uint32_t)(popc(lo ^ glo) + popc(hi ^ ghi) + popc(rlo ^ dlo) + popc(rhi ^ dhi));
+                nn_nnn = dnn | (dnnn << 16);
+                dp = (uint32_t)(popc(xor3(lo, rlo, dlo) ^ glo) + popc(xor3(hi, rhi, dhi) ^ ghi));
+            } else {
+                const uint32_t r = Ll == 32u ? ((lo >> 1) | (lo << 31)) : rot_right_small(lo, Ll);
+                const uint32_t g = Ll == 32u ? ((dlo >> 1) | (dlo << 31)) : rot_right_small(dlo, Ll);
+                nn_nnn = (uint32_t)(popc(lo ^ r) + popc(lo ^ dlo)) | ((uint32_t)(popc(lo ^ g) + popc(r ^ dlo)) << 16);
+                dp = (uint32_t)popc(xor3(lo, r, dlo) ^ g);
+            }
+        }
+        nn_nnn = group_sum(nn_nnn, Ll);
+        dp = group_sum(dp, Ll);
+        if (y == 0u) {
+            uint32_t *c = cntg + l * 4u;
+            c[0] = nn_nnn & 0xFFFFu;
+            c[1] = nn_nnn >> 16;
+            c[2] = dp;
+            c[3] = 0u;
+        }
+        if (l == nl) break;
+        uint32_t blk = 0;
+        if (y < Ll && (y & 1u) == 0u) {
+            if (wide)
+                blk = block16(lo, dlo, coin & 0xFFFFu) | (block16(hi, dhi, coin >> 16) << 16);
+            else
+                blk = block16(lo, dlo, coin) & ((1u << Lb) - 1u);
+        }
+        const uint32_t nlo = (uint32_t)__shfl((int)blk, (int)((2u * y) & (WIDTH - 1u)), (int)WIDTH);
+        coff += level_words(L, l + 1u);
+        lo = y < Lb ? nlo : 0u;
+        hi = 0u;
+    }
+}
+
+template <uint32_t SHAPE>
 __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
                          PackedLattice *__restrict__ lats, uint32_t *lds, Team tm)
 {
     const uint32_t L = (uint32_t)sl.L, G = (uint32_t)sl.G, nl = (uint32_t)sl.n_levels;
     const uint32_t n_chunk = sl.n_sweeps;
     const uint32_t T = tm.T;
+    if constexpr (SHAPE != kShapeGeneric) {
+        // what the shape implies (slot_shape): teams of at least a wave and
+        // L >= 64, so the kernel keeps only those code paths
+        __builtin_assume(T >= 64u);
+        __builtin_assume(L >= 64u);
+    }
     const SlotLayout lay = slot_layout(L, G, nl, T);
     const uint32_t W = lay.W, rs = record_size(nl), nw = lay.team_waves;
     const uint32_t cstride = nw * (nl + 1u) * 4u;
@@ -388,7 +459,7 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
         //      the spins): call c of the flattened list gives coin words
         //      4q..4q+3 of its level
         for (uint32_t c = 0; c < 2u; ++c) {
-            if (T >= 64u)
+            if (SHAPE != kShapeGeneric || T >= 64u)
                 sweep_colour<true>(meta, spins, L, W, G, tm, it, c);
             else
                 sweep_colour<false>(meta, spins, L, W, G, tm, it, c);
@@ -422,6 +493,9 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
         }
         if (!measuring) continue;
 
+        // The sample's sums and products into lattice g's partial record,
+        // entries k = k0, k0 + ks, ...: from the operator values in `sv`, or
+        // each entry forming its two values from the counts.
         // The sample's sums and products into lattice g's partial record,
         // entries k = k0, k0 + ks, ...: from the operator values in `sv`, or
         // each entry forming its two values from the counts.
@@ -506,8 +580,16 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
                 for (uint32_t u = tm.u0; u < T; u += tm.us) {
                     uint32_t *cntg = cnt + g * cstride + (u >> 6) * (nl + 1u) * 4u;
                     if (u < 64u && meas) {
-                        pyramid(lvl + g * lay.lvl_words, coins + g * lay.lvl_words, cntg, L, pl, nl, off,
-                                coin_off, u, T < 64u ? T : 64u);
+                        if constexpr (SHAPE != kShapeGeneric) {
+                            if ((L >> pl) == 64u)
+                                pyramid_t<64u, 64u>(lvl + g * lay.lvl_words, coins + g * lay.lvl_words, cntg, L,
+                                                    pl, nl, off, coin_off, u);
+                            else
+                                pyramid_t<32u, 64u>(lvl + g * lay.lvl_words, coins + g * lay.lvl_words, cntg, L,
+                                                    pl, nl, off, coin_off, u);
+                        } else
+                            pyramid(lvl + g * lay.lvl_words, coins + g * lay.lvl_words, cntg, L, pl, nl, off,
+                                    coin_off, u, T < 64u ? T : 64u);
                         if (tm.block) {
                             // A team spanning waves: its first wave, which holds
                             // the pyramid's counts (the other partials were

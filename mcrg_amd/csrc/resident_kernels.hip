@@ -111,7 +111,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
             tm.us = tm.T;
             tm.lane = threadIdx.x;
             tm.block = tm.T > 64u;
-            run_slot(sl, slot_lats, lats, lds, tm);
+            // Slots of either wide shape run the wide body (slot_shape), the
+            // others the generic body.  Workgroups of 64 and 128 threads keep
+            // the generic body only: with both, their register allocation
+            // spills.
+            if (NT >= 256 && slot_shape((uint32_t)sl.L, (uint32_t)sl.G, (uint32_t)sl.n_levels, (uint32_t)NT) !=
+                                 kShapeGeneric)
+                run_slot<kShapeRows64>(sl, slot_lats, lats, lds, tm);
+            else {
+                // The generic body carries the small lattices: chains of short
+                // dependent steps on a few lanes per lattice, whose slots take
+                // the longest in a mixed launch.  Their waves issue first on a
+                // SIMD they share with wide slots, which fill the cycles left.
+                __builtin_amdgcn_s_setprio(3);
+                run_slot<kShapeGeneric>(sl, slot_lats, lats, lds, tm);
+                __builtin_amdgcn_s_setprio(0);
+            }
         }
         if (n_sub > 1u) {
             // this part's write-back (spins, records, counters) goes to the
