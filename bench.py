@@ -55,30 +55,22 @@ METRIC = "spin-updates/ns (batched vs plain) at matched y_t error; p50/p99 job l
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of k_resident<threads>, keyed by (replicas, sweeps per
 # launch, threads, min_team, max_slot_lds, threads_large, sub_chunks), from two rocprofv3
-# --pmc passes over `bench.py --only batched` with the same configuration flags:
-# FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 KiB per dispatch, the scale factors read
-# off the same counters on a 1 GiB read + 1 GiB write copy kernel
-# (--copy-calibration).  Configurations without a counter run report null.
-# The entries were measured with every slot in one part (sub_chunks 1); the
-# default splits slots into parts, each of which reloads and writes back its
-# lattices, and has no counter run.
-# Changes to the kernel's arithmetic alone leave these unchanged: across the
-# change of the Philox key mix to one v_bitop3_b32 the default configuration
-# read 3,156.1 then 3,157.0 KiB and wrote 5,047.8 then 5,048 KiB.
+# --pmc passes over `bench.py --only batched --sub-chunks 1` (FETCH_SIZE x 2.00
+# + WRITE_SIZE x 1.00 KiB per dispatch, the scale factors read off the same
+# counters on a 1 GiB read + 1 GiB write copy kernel, --copy-calibration).
+# Configurations without a counter run report null.  The passes ran with
+# every slot in one part (sub_chunks 1); the default splits slots into parts,
+# each of which reloads and writes back its lattices, and is not counted.
 MEASURED_TRAFFIC = {
-    (4096, 100, 256, 16, 0, 0, 1): (3156.1 * 2.0 + 5047.8) * 1024.0,
-    # two launch classes here (the L = 8 slots grow past 64 KiB): sums over both
-    (4096, 100, 256, 16, 65536, 0, 1): (3343.1 * 2.0 + 5046.9) * 1024.0,
+    (4096, 100, 256, 16, 0, 0, 1): (3282.0 * 2.0 + 5048.0) * 1024.0,
 }
 # The same kernel is bound by VALU issue, not HBM: rocprofv3 --pmc SQ_INSTS_VALU
-# and GRBM_GUI_ACTIVE over the same launches give the wave-level VALU
+# and GRBM_GUI_ACTIVE over the same one-part launches give the wave-level VALU
 # instructions per launch and the busy cycles per XCD (8 XCDs, 1024 SIMDs that
-# issue one wave64 VALU instruction per 4 cycles).
-# Measured on the kernel with the Philox key mix as one v_bitop3_b32; the
-# blocking and plaquette parity terms have since taken the same form and the
-# register pyramid sums fewer lanes, so the current count is at most this.
+# issue one wave64 VALU instruction per 4 cycles).  Measured on the kernel with
+# the wide slot body; parts add only their reloads.
 MEASURED_VALU = {
-    (4096, 100, 256, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 5.5006e8, "gui_active_cycles_per_xcd": 32599700.0 / 8},
+    (4096, 100, 256, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 4.700e8, "gui_active_cycles_per_xcd": 2.655e7 / 8},
 }
 
 
