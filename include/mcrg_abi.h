@@ -323,8 +323,11 @@ mcrg_status mcrg_server_completed(mcrg_server s, uint64_t *ids, int32_t max_ids,
 /* Runs the scheduler on a thread of its own until mcrg_server_stop. */
 mcrg_status mcrg_server_start(mcrg_server s);
 mcrg_status mcrg_server_stop(mcrg_server s);
-/* One scheduling iteration on the calling thread (when not started):
- * retire, admit, launch one chunk, wait for it.  *active = jobs resident. */
+/* One scheduling iteration on the calling thread (when not started): admit,
+ * queue one chunk, then wait for the previous chunk and retire the jobs it
+ * finished (a job completes in the iteration after its last chunk, while the
+ * next chunk runs).  *active = jobs resident, finished ones not yet retired
+ * included. */
 mcrg_status mcrg_server_step(mcrg_server s, int32_t *active);
 mcrg_status mcrg_server_stats_get(mcrg_server s, mcrg_server_stats *out);
 

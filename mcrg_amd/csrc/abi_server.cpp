@@ -2,10 +2,13 @@
 //
 // One scheduler (a thread of its own after mcrg_server_start, or the caller of
 // mcrg_server_step) owns the engine and the device stream.  Each iteration:
-//   retire  jobs the previous launch finished: records to host, memory freed
 //   admit   queued jobs in arrival order while the device-memory and LDS
 //           budgets have room
-//   launch  one time slice for every resident job, then wait for it
+//   launch  one time slice for every resident job that still has sweeps
+//   retire  the jobs the previous launch finished: wait for that launch
+//           alone, records to host, memory freed
+// so the next launch is already queued while a launch's jobs are retired and
+// new ones admitted, and the device does not wait for the host.
 // Submitting, polling and reading results only touch the host job table.
 #include <algorithm>
 #include <atomic>
@@ -32,6 +35,7 @@ struct ServerJob {
     int n_bins = 0;
     uint32_t lds = 0;  // LDS share while resident
     bool cancel = false;
+    bool retiring = false;  // finished by the last launch, records not read yet
     double t_submit = 0.0, t_start = 0.0, t_finish = 0.0;
     std::vector<int64_t> records;
 };
@@ -58,6 +62,9 @@ struct mcrg_server_s {
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     mcrg_server_stats st{};
     std::string fatal;
+    // jobs the launch `retire_launch` finished, retired after the next launch
+    std::vector<uint64_t> retire;
+    uint64_t retire_launch = 0;
 
     double now() const
     {
@@ -123,22 +130,22 @@ static mcrg_status server_step(mcrg_server s, int32_t *active)
         if (active) *active = 0;
         return MCRG_OK;
     }
-    // ---- one chunk for every resident job
+    // ---- one chunk for every resident job that still has sweeps (the jobs
+    //      waiting for retirement have none); not waited for here
     const double t_launch = s->now();
     uint32_t adv = 0;
     KernelTimer *tm = s->ctx->timer.enabled ? &s->ctx->timer : nullptr;
+    const uint64_t before = s->engine.launches();
     hipError_t e = s->engine.tick((uint32_t)s->eo.sweeps_per_launch, adv, tm);
     (void)adv;
-    const double t_wait = s->now();
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    const double t_done = s->now();
     if (e != hipSuccess) {
         set_error(std::string("server launch (") + s->engine.failed_call() + "): " + hipGetErrorString(e));
         return MCRG_ERR_DEVICE;
     }
-    // ---- retire finished jobs.  The job table is shared with client threads
-    //      (submit inserts, release erases): it is only touched under `mu`;
-    //      device reads happen outside the lock.
+    const bool launched = s->engine.launches() != before;
+    // ---- retire the jobs the previous launch finished.  The job table is
+    //      shared with client threads (submit inserts, release erases): it is
+    //      only touched under `mu`; device reads happen outside the lock.
     struct Done {
         uint64_t id;
         int engine_id;
@@ -148,9 +155,9 @@ static mcrg_status server_step(mcrg_server s, int32_t *active)
     std::vector<Done> done;
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        for (uint64_t id : s->running) {
+        for (uint64_t id : s->retire) {
             auto it = s->jobs.find(id);
-            if (it == s->jobs.end() || !s->engine.finished(it->second.engine_id)) continue;
+            if (it == s->jobs.end() || it->second.state != MCRG_JOB_RUNNING) continue;  // cancelled meanwhile
             const ServerJob &j = it->second;
             Done d;
             d.id = id;
@@ -159,6 +166,14 @@ static mcrg_status server_step(mcrg_server s, int32_t *active)
             d.updates = (double)j.spec.L * j.spec.L * (double)(j.spec.n_therm + j.spec.n_meas);
             done.push_back(std::move(d));
         }
+        s->retire.clear();
+    }
+    const double t_wait = s->now();
+    // at most two launches in flight: the one just queued and the one before,
+    // which this iteration waits for (its finished jobs are read back now)
+    if (launched && s->engine.launches() >= 2u && (e = s->engine.wait_launch(s->engine.launches() - 2u)) != hipSuccess) {
+        set_error(std::string("server launch: ") + hipGetErrorString(e));
+        return MCRG_ERR_DEVICE;
     }
     if (!done.empty()) {
         std::vector<int> ids;
@@ -169,13 +184,14 @@ static mcrg_status server_step(mcrg_server s, int32_t *active)
             dst.push_back(d.rec.data());
             ns.push_back((int64_t)d.rec.size());
         }
-        e = s->engine.read_records_many(ids, dst, ns);
+        e = s->engine.read_records_after(s->retire_launch, ids, dst, ns);
         if (e != hipSuccess) {
-            set_error(std::string("server readback: ") + hipGetErrorString(e));
+            set_error(std::string("server readback (") + s->engine.failed_call() + "): " + hipGetErrorString(e));
             return MCRG_ERR_DEVICE;
         }
         for (Done &d : done) s->engine.remove(d.engine_id);
     }
+    const double t_done = s->now();
     {
         std::lock_guard<std::mutex> lk(s->mu);
         const double t = s->now();
@@ -199,7 +215,19 @@ static mcrg_status server_step(mcrg_server s, int32_t *active)
             }
             s->running.resize(k);
         }
-        s->st.launches++;
+        // the jobs this launch finishes (the host mirrors the device counters
+        // exactly), retired in the next iteration once it has run
+        if (launched) {
+            for (uint64_t id : s->running) {
+                ServerJob &j = s->jobs.at(id);  // running jobs are never erased
+                if (!j.retiring && s->engine.finished(j.engine_id)) {
+                    j.retiring = true;
+                    s->retire.push_back(id);
+                }
+            }
+            s->retire_launch = s->engine.launches() - 1u;
+            s->st.launches++;
+        }
         s->st.busy_s += s->now() - t_launch;
         s->st.device_wait_s += t_done - t_wait;
         s->st.running = (int64_t)s->running.size();

@@ -70,6 +70,44 @@ void Arena::free(size_t off, size_t bytes)
     }
 }
 
+// ------------------------------------------------------------------ PinnedPair
+PinnedPair::~PinnedPair()
+{
+    for (int i = 0; i < 2; ++i) {
+        if (ev_[i]) {
+            (void)hipEventSynchronize(ev_[i]);
+            (void)hipEventDestroy(ev_[i]);
+        }
+        if (buf_[i]) (void)hipHostFree(buf_[i]);
+    }
+}
+
+void *PinnedPair::next(size_t bytes, hipError_t &e)
+{
+    e = hipSuccess;
+    const int i = cur_ ^ 1;
+    if (!ev_[i] && (e = hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming)) != hipSuccess) return nullptr;
+    // the copies that read this buffer last have to be done before it is rewritten
+    if (pending_[i] && (e = hipEventSynchronize(ev_[i])) != hipSuccess) return nullptr;
+    pending_[i] = false;
+    if (bytes > cap_[i]) {
+        if (buf_[i]) (void)hipHostFree(buf_[i]);
+        buf_[i] = nullptr;
+        cap_[i] = 0;
+        const size_t cap = std::max<size_t>(bytes, 1u << 16);
+        if ((e = hipHostMalloc(&buf_[i], cap, hipHostMallocDefault)) != hipSuccess) return nullptr;
+        cap_[i] = cap;
+    }
+    cur_ = i;
+    return buf_[i];
+}
+
+hipError_t PinnedPair::used(hipStream_t s)
+{
+    pending_[cur_] = true;
+    return hipEventRecord(ev_[cur_], s);
+}
+
 // ------------------------------------------------------------------ engine
 ResidentEngine::~ResidentEngine()
 {
@@ -93,9 +131,14 @@ ResidentEngine::~ResidentEngine()
     if (d_err_) (void)hipFree(d_err_);
     if (d_init_idx_) (void)hipFree(d_init_idx_);
     if (d_stage_) (void)hipFree(d_stage_);
-    if (h_stage_) (void)hipHostFree(h_stage_);
-    if (h_stage_idx_) (void)hipHostFree(h_stage_idx_);
     if (h_rec_) (void)hipHostFree(h_rec_);
+    if (h_err_) (void)hipHostFree(h_err_);
+    for (hipEvent_t e : ev_launch_)
+        if (e) (void)hipEventDestroy(e);
+    if (rb_) {
+        (void)hipStreamSynchronize(rb_);
+        (void)hipStreamDestroy(rb_);
+    }
 }
 
 hipError_t ResidentEngine::init(int device, size_t arena_bytes, int capacity, const EngineOptions &opt,
@@ -119,6 +162,12 @@ hipError_t ResidentEngine::init(int device, size_t arena_bytes, int capacity, co
     if ((e = hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming)) != hipSuccess) return e;
     for (hipEvent_t &ev : ev_join_)
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
+    for (hipEvent_t &ev : ev_launch_)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipStreamCreateWithFlags(&rb_, hipStreamNonBlocking)) != hipSuccess) return e;
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&h_err_), sizeof(uint32_t), hipHostMallocDefault)) !=
+        hipSuccess)
+        return e;
     return hipSuccess;
 }
 
@@ -287,29 +336,20 @@ hipError_t ResidentEngine::flush_pending()
     if (pending_init_.empty()) return hipSuccess;
     hipError_t e;
     // descriptors of the new lattices staged in pinned memory, two copies and
-    // one admission launch for all of them (stream order puts it after any
-    // launch that still reads the table; new ids are in no slot)
+    // one admission launch for all of them (stream order puts them after any
+    // launch that still reads the table; new ids are in no slot).  The staging
+    // buffers are used in turn, so the host does not wait for those launches.
     const size_t n = pending_init_.size();
-    if (n > stage_cap_) {
-        if (h_stage_) (void)hipHostFree(h_stage_);
-        if (h_stage_idx_) (void)hipHostFree(h_stage_idx_);
-        h_stage_ = nullptr;
-        h_stage_idx_ = nullptr;
-        stage_cap_ = 0;
-        const size_t cap = std::max<size_t>(n, 256);
-        if ((e = hipHostMalloc(reinterpret_cast<void **>(&h_stage_), cap * sizeof(PackedLattice),
-                               hipHostMallocDefault)) != hipSuccess)
-            return e;
-        if ((e = hipHostMalloc(reinterpret_cast<void **>(&h_stage_idx_), cap * sizeof(int32_t),
-                               hipHostMallocDefault)) != hipSuccess)
-            return e;
-        stage_cap_ = cap;
-    }
+    const size_t idx_off = round_up(sizeof(PackedLattice) * n, 256);
+    char *buf = static_cast<char *>(stage_.next(idx_off + sizeof(int32_t) * n, e));
+    if (!buf) return e;
+    PackedLattice *h_stage = reinterpret_cast<PackedLattice *>(buf);
+    int32_t *h_idx = reinterpret_cast<int32_t *>(buf + idx_off);
     for (size_t k = 0; k < n; ++k) {
         const int id = pending_init_[k];
         const Job &job = jobs_[(size_t)id];
-        h_stage_idx_[k] = id;
-        PackedLattice &P = h_stage_[k];
+        h_idx[k] = id;
+        PackedLattice &P = h_stage[k];
         std::memset(&P, 0, sizeof(P));
         const uint32_t W = cb_words((uint32_t)job.spec.L);
         P.spins = reinterpret_cast<uint32_t *>(arena_.base() + job.off);
@@ -326,15 +366,13 @@ hipError_t ResidentEngine::flush_pending()
         P.sweeps_per_bin = job.spec.sweeps_per_bin;
         P.max_bins = job.n_bins;
     }
-    if ((e = hipMemcpyAsync(d_stage_, h_stage_, sizeof(PackedLattice) * n, hipMemcpyHostToDevice, stream_)) !=
+    if ((e = hipMemcpyAsync(d_stage_, h_stage, sizeof(PackedLattice) * n, hipMemcpyHostToDevice, stream_)) !=
         hipSuccess)
         return e;
-    if ((e = hipMemcpyAsync(d_init_idx_, h_stage_idx_, sizeof(int32_t) * n, hipMemcpyHostToDevice, stream_)) !=
-        hipSuccess)
+    if ((e = hipMemcpyAsync(d_init_idx_, h_idx, sizeof(int32_t) * n, hipMemcpyHostToDevice, stream_)) != hipSuccess)
         return e;
+    if ((e = stage_.used(stream_)) != hipSuccess) return e;
     if ((e = launch_packed_init(d_lats_, d_stage_, d_init_idx_, (int)n, stream_)) != hipSuccess) return e;
-    // the pinned staging is rewritten by the next admission
-    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
     pending_init_.clear();
     return hipSuccess;
 }
@@ -439,7 +477,10 @@ void ResidentEngine::rebuild_slots(uint32_t chunk)
 hipError_t ResidentEngine::upload_slots()
 {
     hipError_t e;
-    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
+    // Growing a device table frees the old one, which a queued launch may
+    // still read: only then does the host wait for the stream.
+    if (slots_.size() > slot_cap_ || slot_lats_.size() > slot_lat_cap_ || slots_.size() > done_cap_)
+        if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
     if (slots_.size() > slot_cap_) {
         if (d_slots_) (void)hipFree(d_slots_);
         slot_cap_ = std::max<size_t>(slots_.size(), 2 * slot_cap_);
@@ -451,12 +492,20 @@ hipError_t ResidentEngine::upload_slots()
         if ((e = hipMalloc(&d_slot_lats_, sizeof(int32_t) * slot_lat_cap_)) != hipSuccess) return e;
     }
     if (!slots_.empty()) {
-        if ((e = hipMemcpy(d_slots_, slots_.data(), sizeof(Slot) * slots_.size(), hipMemcpyHostToDevice)) !=
+        // copies on the stream, after the launches that read the old tables,
+        // from pinned staging used in turn
+        const size_t lat_off = round_up(sizeof(Slot) * slots_.size(), 256);
+        char *buf = static_cast<char *>(slot_stage_.next(lat_off + sizeof(int32_t) * slot_lats_.size(), e));
+        if (!buf) return e;
+        std::memcpy(buf, slots_.data(), sizeof(Slot) * slots_.size());
+        std::memcpy(buf + lat_off, slot_lats_.data(), sizeof(int32_t) * slot_lats_.size());
+        if ((e = hipMemcpyAsync(d_slots_, buf, sizeof(Slot) * slots_.size(), hipMemcpyHostToDevice, stream_)) !=
             hipSuccess)
             return e;
-        if ((e = hipMemcpy(d_slot_lats_, slot_lats_.data(), sizeof(int32_t) * slot_lats_.size(),
-                           hipMemcpyHostToDevice)) != hipSuccess)
+        if ((e = hipMemcpyAsync(d_slot_lats_, buf + lat_off, sizeof(int32_t) * slot_lats_.size(),
+                                hipMemcpyHostToDevice, stream_)) != hipSuccess)
             return e;
+        if ((e = slot_stage_.used(stream_)) != hipSuccess) return e;
     }
     if (slots_.size() > done_cap_) {
         if (d_done_) (void)hipFree(d_done_);
@@ -493,22 +542,18 @@ hipError_t ResidentEngine::upload_slots()
     return hipSuccess;
 }
 
-hipError_t ResidentEngine::check_parts()
+// The error word of the launches, read on stream `s` after the copies the
+// caller queued there (one wait for both).  A part that gave up its wait set
+// it; it is reported once and cleared, so that later launches are judged alone.
+hipError_t ResidentEngine::check_parts(hipStream_t s)
 {
-    if (!parted_) return hipSuccess;
-    // on the engine's stream, which the caller has just drained: no wait on
-    // other streams
-    uint32_t err = 0;
     hipError_t e;
-    if ((e = hipMemcpyAsync(&err, d_err_, sizeof(err), hipMemcpyDeviceToHost, stream_)) != hipSuccess) return e;
-    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
-    parted_ = false;
-    if (err != 0u) {
-        // reported once: the word is cleared (a plain copy on the drained
-        // stream, outside any graph) so that later launches are judged alone
+    if ((e = hipMemcpyAsync(h_err_, d_err_, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    if (*h_err_ != 0u) {
         static const uint32_t zero = 0u;
-        if ((e = hipMemcpyAsync(d_err_, &zero, sizeof(zero), hipMemcpyHostToDevice, stream_)) != hipSuccess) return e;
-        if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(d_err_, &zero, sizeof(zero), hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
         failed_ = "k_resident: a slot part waited for its predecessor beyond the limit";
         return hipErrorLaunchFailure;
     }
@@ -527,7 +572,6 @@ hipError_t ResidentEngine::launch_classes(KernelTimer *tm)
         hipStream_t s = c == 0 ? stream_ : aux_[c - 1];
         if (c > 0 && (e = hipStreamWaitEvent(s, ev_fork_, 0)) != hipSuccess) return e;
         // one queue counter per class, 64 bytes apart
-        if (lc.n_sub > 1u) parted_ = true;
         if ((e = launch_resident(lc.threads, d_slots_ + lc.first, lc.count, lc.n_sub, d_slot_lats_, d_lats_,
                                  d_queue_ + 16u * c, d_done_ + lc.first, d_err_, lc.lds, (uint32_t)lc.grid, s)) !=
             hipSuccess) {
@@ -581,6 +625,8 @@ hipError_t ResidentEngine::tick(uint32_t n, uint32_t &advanced, KernelTimer *tm)
         return e;
     }
     if (timing && (e = hipEventRecord(tune_b_, stream_)) != hipSuccess) return e;
+    if ((e = hipEventRecord(ev_launch_[launches_ & 1u], stream_)) != hipSuccess) return e;
+    ++launches_;
     // mirror the device counters
     bool any_done = false;
     double updates = 0.0;
@@ -639,8 +685,7 @@ hipError_t ResidentEngine::read_spins(int id, int8_t *host)
     if ((e = hipMemcpyAsync(words.data(), arena_.base() + job.off, words.size() * 4u, hipMemcpyDeviceToHost,
                             stream_)) != hipSuccess)
         return e;
-    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
-    if ((e = check_parts()) != hipSuccess) return e;
+    if ((e = check_parts(stream_)) != hipSuccess) return e;
     for (uint32_t c = 0; c < 2u; ++c)
         for (uint32_t w = 0; w < W; ++w)
             for (uint32_t b = 0; b < 32u; ++b) {
@@ -652,11 +697,10 @@ hipError_t ResidentEngine::read_spins(int id, int8_t *host)
     return hipSuccess;
 }
 
-hipError_t ResidentEngine::read_records_many(const std::vector<int> &ids, const std::vector<int64_t *> &hosts,
-                                             const std::vector<int64_t> &ns)
+hipError_t ResidentEngine::copy_records(hipStream_t s, const std::vector<int> &ids,
+                                        const std::vector<int64_t *> &hosts, const std::vector<int64_t> &ns)
 {
     hipError_t e;
-    if ((e = flush_pending()) != hipSuccess) return e;
     size_t total = 0;
     for (int64_t n : ns) total += (size_t)n;
     if (total > rec_cap_) {
@@ -668,24 +712,48 @@ hipError_t ResidentEngine::read_records_many(const std::vector<int> &ids, const 
             return e;
         rec_cap_ = cap;
     }
-    // every copy queued into pinned memory, one wait for all of them
+    // every copy queued into pinned memory, one wait for all of them (and the
+    // error word)
     size_t off = 0;
     for (size_t k = 0; k < ids.size(); ++k) {
         const Job &job = jobs_[(size_t)ids[k]];
         const uint32_t W = cb_words((uint32_t)job.spec.L);
         const char *rec = arena_.base() + job.off + round_up(2u * W * 4u, 256);
-        if ((e = hipMemcpyAsync(h_rec_ + off, rec, (size_t)ns[k] * 8u, hipMemcpyDeviceToHost, stream_)) != hipSuccess)
+        if ((e = hipMemcpyAsync(h_rec_ + off, rec, (size_t)ns[k] * 8u, hipMemcpyDeviceToHost, s)) != hipSuccess)
             return e;
         off += (size_t)ns[k];
     }
-    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
-    if ((e = check_parts()) != hipSuccess) return e;
+    if ((e = check_parts(s)) != hipSuccess) return e;
     off = 0;
     for (size_t k = 0; k < ids.size(); ++k) {
         std::memcpy(hosts[k], h_rec_ + off, (size_t)ns[k] * 8u);
         off += (size_t)ns[k];
     }
     return hipSuccess;
+}
+
+hipError_t ResidentEngine::read_records_many(const std::vector<int> &ids, const std::vector<int64_t *> &hosts,
+                                             const std::vector<int64_t> &ns)
+{
+    hipError_t e;
+    if ((e = flush_pending()) != hipSuccess) return e;
+    return copy_records(stream_, ids, hosts, ns);
+}
+
+hipError_t ResidentEngine::wait_launch(uint64_t k)
+{
+    if (k >= launches_ || launches_ - k > 2u) return hipErrorInvalidValue;
+    return hipEventSynchronize(ev_launch_[k & 1u]);
+}
+
+hipError_t ResidentEngine::read_records_after(uint64_t k, const std::vector<int> &ids,
+                                              const std::vector<int64_t *> &hosts, const std::vector<int64_t> &ns)
+{
+    hipError_t e;
+    // launch k (the event behind it; later launches keep running) ...
+    if ((e = wait_launch(k)) != hipSuccess) return e;
+    // ... then the copies on the readback stream, which no launch waits behind
+    return copy_records(rb_, ids, hosts, ns);
 }
 
 hipError_t ResidentEngine::read_records(int id, int64_t *host, int64_t n)
@@ -696,8 +764,7 @@ hipError_t ResidentEngine::read_records(int id, int64_t *host, int64_t n)
     const uint32_t W = cb_words((uint32_t)job.spec.L);
     const char *rec = arena_.base() + job.off + round_up(2u * W * 4u, 256);
     if ((e = hipMemcpyAsync(host, rec, (size_t)n * 8u, hipMemcpyDeviceToHost, stream_)) != hipSuccess) return e;
-    if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return e;
-    return check_parts();
+    return check_parts(stream_);
 }
 
 }  // namespace mcrg

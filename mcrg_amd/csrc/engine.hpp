@@ -85,6 +85,26 @@ struct EngineOptions {
     int sub_chunks = 0;
 };
 
+// Two pinned host buffers used in turn as the source of asynchronous copies:
+// a buffer is handed out again only after the event recorded behind its last
+// copy has completed, so the host never rewrites memory a copy still reads
+// and never waits for the launches queued after that copy.
+class PinnedPair {
+  public:
+    ~PinnedPair();
+    // The next buffer, at least `bytes` long (nullptr on error, in `e`).
+    void *next(size_t bytes, hipError_t &e);
+    // Marks the buffer handed out last as read by copies queued on `s`.
+    hipError_t used(hipStream_t s);
+
+  private:
+    void *buf_[2] = {nullptr, nullptr};
+    size_t cap_[2] = {0, 0};
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    bool pending_[2] = {false, false};
+    int cur_ = 1;
+};
+
 class ResidentEngine {
   public:
     ~ResidentEngine();
@@ -116,6 +136,17 @@ class ResidentEngine {
     // Records of several lattices through one pinned staging buffer and one wait.
     hipError_t read_records_many(const std::vector<int> &ids, const std::vector<int64_t *> &hosts,
                                  const std::vector<int64_t> &ns);
+    // Launches enqueued so far; launch k (0-based) is launches() - 1 right after
+    // the tick that enqueued it.
+    uint64_t launches() const { return launches_; }
+    // Waits for launch k (one of the last two) and what was queued before it.
+    hipError_t wait_launch(uint64_t k);
+    // read_records_many for lattices that no launch after launch k touches
+    // (finished by k or earlier): waits for launch k alone and copies on the
+    // engine's readback stream, so later launches keep running meanwhile.
+    // Launch k must be one of the last two.
+    hipError_t read_records_after(uint64_t k, const std::vector<int> &ids, const std::vector<int64_t *> &hosts,
+                                  const std::vector<int64_t> &ns);
     void state(int id, uint32_t &t, uint32_t &n_measured) const;
     const JobSpec &spec(int id) const { return jobs_[id].spec; }
     int n_bins(int id) const { return jobs_[id].n_bins; }
@@ -210,19 +241,26 @@ class ResidentEngine {
     uint32_t *d_done_ = nullptr;  // per-slot progress of parted launches
     size_t done_cap_ = 0;
     uint32_t *d_err_ = nullptr;   // set by a part whose wait timed out
-    bool parted_ = false;         // a parted launch ran since the last check
+    uint32_t *h_err_ = nullptr;   // its pinned host copy
     int cus_ = 0;
-    hipError_t check_parts();
+    hipError_t check_parts(hipStream_t s);
     int32_t *d_init_idx_ = nullptr;
     bool slots_dirty_ = true;
     uint32_t slots_chunk_ = 0;
     std::vector<int32_t> pending_init_;
-    // admission staging (pinned) and its device copy; record readback staging
-    PackedLattice *h_stage_ = nullptr, *d_stage_ = nullptr;
-    int32_t *h_stage_idx_ = nullptr;
-    size_t stage_cap_ = 0;
+    // admission staging (pinned, in turn) and its device copy; slot-table
+    // staging; record readback staging
+    PinnedPair stage_;
+    PackedLattice *d_stage_ = nullptr;
+    PinnedPair slot_stage_;
     int64_t *h_rec_ = nullptr;
     size_t rec_cap_ = 0;
+    // one event behind each launch (the last two), the readback stream
+    hipEvent_t ev_launch_[2] = {nullptr, nullptr};
+    uint64_t launches_ = 0;
+    hipStream_t rb_ = nullptr;
+    hipError_t copy_records(hipStream_t s, const std::vector<int> &ids, const std::vector<int64_t *> &hosts,
+                            const std::vector<int64_t> &ns);
     const char *failed_ = "";
 };
 
