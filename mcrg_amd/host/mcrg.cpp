@@ -530,8 +530,30 @@ double zero_crossing(const std::vector<double> &J_in, const std::vector<double> 
             w.push_back(w_in[i]);
         }
     const size_t n = J.size();
-    // the sign change between neighbours (in J) whose signs are the most
-    // significant, |D| sqrt(w) standard errors
+    // The weighted straight line through every point whenever it fits (chi2
+    // within 3 standard deviations of its mean): in narrow brackets D(J) is
+    // straight and noise decides the rest, and the line's zero, unlike a fit
+    // confined to the bracket, follows the data out of it.
+    if (n >= 3) {
+        double S = 0, Sx = 0, Sy = 0, Sxx = 0, Sxy = 0;
+        for (size_t i = 0; i < n; ++i) {
+            S += w[i];
+            Sx += w[i] * J[i];
+            Sy += w[i] * D[i];
+            Sxx += w[i] * J[i] * J[i];
+            Sxy += w[i] * J[i] * D[i];
+        }
+        const double det = S * Sxx - Sx * Sx;
+        if (det != 0.0) {
+            const double b = (S * Sxy - Sx * Sy) / det, a = (Sy - b * Sx) / S;
+            double chi2 = 0.0;
+            for (size_t i = 0; i < n; ++i) chi2 += w[i] * (D[i] - a - b * J[i]) * (D[i] - a - b * J[i]);
+            const double dof = (double)n - 2.0;
+            if (b != 0.0 && chi2 <= dof + 3.0 * std::sqrt(2.0 * dof)) return -a / b;
+        }
+    }
+    // Otherwise D bends over the bracket: the sign change between neighbours
+    // (in J) whose signs are the most significant, |D| sqrt(w) standard errors
     std::vector<size_t> byJ(n);
     for (size_t i = 0; i < n; ++i) byJ[i] = i;
     std::sort(byJ.begin(), byJ.end(), [&](size_t x, size_t y) { return J[x] < J[y]; });

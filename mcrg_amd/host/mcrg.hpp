@@ -82,15 +82,17 @@ std::string rounded_str(double num, int precision);  // get_rounded_str, src/def
 bool write_iter(int i);                                // src/definitions.cpp:46-68
 int split_samples(int rank, int n_processes, int n_samples);  // src/definitions.cpp:79-87
 // Zero of D(J) sampled at couplings J with weights w (1 / variance), as the
-// K-bisection locates it.  Over a wide bracket D(J) bends (it saturates on
-// either side of the critical coupling), so a straight line through every
-// point is biased.  Instead, among the points with positive weight: the sign
-// change between neighbours whose signs are the most significant (|D| sqrt(w)),
-// then the widest window of points nearest it whose weighted quadratic fits
-// (chi^2 within three standard deviations of its mean) and crosses zero inside
-// the window, that zero nearest the sign change; failing that, the cubic
-// through the four nearest points.  Without a sign change: the weighted
-// straight line through every point, an extrapolation.  NaN when undefined.
+// K-bisection locates it.  The weighted straight line through every point when
+// it fits (chi^2 within three standard deviations of its mean): its zero may
+// lie outside the bracket, so a noisy narrow bracket does not pin the answer
+// to its own centre.  Over a wide bracket D(J) bends (it saturates on either
+// side of the critical coupling) and the line is rejected; then, among the
+// points with positive weight: the sign change between neighbours whose signs
+// are the most significant (|D| sqrt(w)), the widest window of points nearest
+// it whose weighted quadratic fits and crosses zero inside the window, that
+// zero nearest the sign change; failing that, the cubic through the four
+// nearest points.  Without a sign change: the straight line, an
+// extrapolation.  NaN when undefined.
 double zero_crossing(const std::vector<double> &J, const std::vector<double> &D, const std::vector<double> &w);
 }  // namespace mcrg_cpp
 

@@ -81,11 +81,24 @@ def test_noisy_sigmoid(seed):
     assert got == pytest.approx(JC, abs=2e-5)
 
 
-def test_narrow_bracket_near_linear():
+def test_narrow_bracket_takes_the_line():
+    """A bracket in which D(J) is straight within its errors gets the straight
+    line's zero, even outside the bracket (a fit confined to the bracket would
+    pin it there).  sd = 1e-3 and slope 1.3 / 0.003 make the zero's
+    statistical error about 2.3e-6; the line's curvature bias here is far
+    below it."""
     J = grid(JC - 1e-4, JC + 3e-4)
     t = np.tanh((J - JC) / 0.003)
     D = t * (1.3 + 0.3 * t)
-    assert crossing(J, D, np.full_like(J, 1e6)) == pytest.approx(JC, abs=1e-7)
+    w = np.full_like(J, 1e6)
+    assert crossing(J, D, w) == pytest.approx(line_crossing(J, D, w), abs=1e-12)
+    assert crossing(J, D, w) == pytest.approx(JC, abs=5e-7)
+    # a bracket entirely above the zero: the line extrapolates below it
+    J2 = grid(JC + 1e-4, JC + 5e-4)
+    t2 = np.tanh((J2 - JC) / 0.003)
+    got = crossing(J2, t2 * (1.3 + 0.3 * t2), w)
+    assert got < J2.min()
+    assert got == pytest.approx(JC, abs=5e-6)
 
 
 @pytest.mark.parametrize("bracket", [(0.43, 0.45), (0.438, 0.443), (0.4402, 0.4412)])
