@@ -61,8 +61,17 @@ HBM_PEAK_GBS = 8000.0
 # Configurations without a counter run report null.  The passes ran with
 # every slot in one part (sub_chunks 1); the default splits slots into parts,
 # each of which reloads and writes back its lattices, and is not counted.
+# Every configuration the autotuner can keep has its passes (threads, min_team,
+# max_slot_lds); threads_large only concerns L >= 512 and leaves this manifest's
+# slots as they are.  A launch of two classes (max_slot_lds 65536) sums both.
 MEASURED_TRAFFIC = {
     (4096, 100, 256, 16, 0, 0, 1): (3282.0 * 2.0 + 5048.0) * 1024.0,
+    (4096, 100, 256, 16, 0, 512, 1): (3282.0 * 2.0 + 5048.0) * 1024.0,
+    (4096, 100, 256, 32, 0, 0, 1): (3285.7 * 2.0 + 5047.8) * 1024.0,
+    (4096, 100, 256, 8, 0, 0, 1): (3284.3 * 2.0 + 5047.8) * 1024.0,
+    (4096, 100, 128, 16, 0, 0, 1): (3176.6 * 2.0 + 5066.3) * 1024.0,
+    (4096, 100, 512, 16, 0, 0, 1): (3267.4 * 2.0 + 5012.3) * 1024.0,
+    (4096, 100, 256, 16, 65536, 0, 1): (2.0 * 1746.6 * 2.0 + 2.0 * 2523.5) * 1024.0,
 }
 # The same kernel is bound by VALU issue, not HBM: rocprofv3 --pmc SQ_INSTS_VALU
 # and GRBM_GUI_ACTIVE over the same one-part launches give the wave-level VALU
@@ -71,6 +80,13 @@ MEASURED_TRAFFIC = {
 # the wide slot body; parts add only their reloads.
 MEASURED_VALU = {
     (4096, 100, 256, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 4.700e8, "gui_active_cycles_per_xcd": 2.655e7 / 8},
+    (4096, 100, 256, 16, 0, 512, 1): {"valu_wave_instr_per_launch": 4.700e8, "gui_active_cycles_per_xcd": 2.655e7 / 8},
+    (4096, 100, 256, 32, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6999e8, "gui_active_cycles_per_xcd": 2.681e7 / 8},
+    (4096, 100, 256, 8, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6999e8, "gui_active_cycles_per_xcd": 2.6785e7 / 8},
+    (4096, 100, 128, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 5.0055e8, "gui_active_cycles_per_xcd": 3.5332e7 / 8},
+    (4096, 100, 512, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 4.7003e8, "gui_active_cycles_per_xcd": 3.8077e7 / 8},
+    (4096, 100, 256, 16, 65536, 0, 1): {"valu_wave_instr_per_launch": 2.0 * 2.3242e8,
+                                        "gui_active_cycles_per_xcd": 2.5244e7 / 8},
 }
 
 
@@ -552,6 +568,9 @@ def main():
                          avg_ms=avg_ms,
                          hbm={"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": achieved / HBM_PEAK_GBS, "bytes_per_launch": bpl,
+                              "counted_bytes_one_part": MEASURED_TRAFFIC.get(cfg_key[:-1] + (1,)),
+                              "counted_basis": "rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE on the same configuration "
+                                               "with every slot in one part (parts add their reloads)",
                               "basis": "algorithmic bytes (3/8 B per site update + 5/24 B per site per measured "
                                        "sweep, bit-packed streaming model)"}),
         "plain_mixed": plain_mixed,
