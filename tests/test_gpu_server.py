@@ -226,3 +226,37 @@ def test_errors_and_cancel(amd, ctx):
         assert srv.poll(long_id)["state"] == amd.JOB_CANCELLED
         st = srv.stats()
         assert st["cancelled"] == 2 and st["hbm_in_use"] == 0 and st["running"] == 0
+
+
+def test_cancel_after_last_launch_before_retirement(amd, ctx, orc):
+    """A job's last launch is queued before its records are read back (the
+    next iteration retires it): it still reads as running after that step and
+    completes in the next one.  Cancelled in between, it never completes, and a
+    job admitted into the memory it frees, beside one that keeps running, both
+    match the oracle."""
+    short = dict(L=32, coupling=KC, seed=11, n_therm=0, n_meas=5, n_levels=2, sweeps_per_bin=5)
+    longer = dict(L=32, coupling=0.4320459, seed=12, n_therm=3, n_meas=40, n_levels=2, sweeps_per_bin=4)
+    later = dict(L=32, coupling=0.4688157, seed=13, n_therm=2, n_meas=12, n_levels=3, sweeps_per_bin=3)
+    with amd.Server(ctx, sweeps_per_launch=5) as srv:
+        a = srv.submit(short)
+        b = srv.submit(longer)
+        srv.step()                      # a's only launch; b's first
+        assert srv.poll(a)["state"] == amd.JOB_RUNNING
+        srv.release(a)                  # cancelled after its last launch
+        c = srv.submit(later)
+        while srv.step() > 0:
+            pass
+        assert srv.poll(a)["state"] == amd.JOB_CANCELLED
+        for jid, job in ((b, longer), (c, later)):
+            assert srv.poll(jid)["state"] == amd.JOB_DONE
+            assert np.array_equal(srv.records(jid).reshape(-1), _expect(orc, job).reshape(-1))
+        st = srv.stats()
+        assert st["cancelled"] == 1 and st["completed"] == 2 and st["hbm_in_use"] == 0
+
+    with amd.Server(ctx, sweeps_per_launch=5) as srv:
+        a = srv.submit(short)
+        srv.step()                      # queued and run; records not read yet
+        assert srv.poll(a)["state"] == amd.JOB_RUNNING
+        assert srv.step() == 0          # retired
+        assert srv.poll(a)["state"] == amd.JOB_DONE
+        assert np.array_equal(srv.records(a).reshape(-1), _expect(orc, short).reshape(-1))
