@@ -35,6 +35,10 @@ constexpr uint32_t kStripCntWords = 64;    // per-slot operator counts, 4 per le
 // A wait gives up (and sets the error word) after 2 s of the 100 MHz clock.
 constexpr uint64_t kStripWaitTicks = 200000000ull;
 constexpr uint32_t kStripMaxWg = 256;
+// With halo rows a sample's first rows are read two sweeps after they are
+// sent, while the sender may already send the next samples': one slot per
+// sample of the last four.
+constexpr uint32_t kStripFirstSlots = 4;
 // LDS reduction scratch: 4 totals, 4 words per wave of up to 16 waves, then
 // a wait word.
 constexpr uint32_t kRedWords = 72, kRedWait = 68;
@@ -46,12 +50,20 @@ struct StripShape {
     // sample is measured from a copy of the spins during the next sweep's
     // colour-0 exchange (the waits of small strips are longer than their work)
     uint32_t overlap;
+    // two halo rows on each side (rows y0-2 .. y0+R+1 in LDS) and one exchange
+    // per sweep: a strip updates colour 0 on its own rows, receives the
+    // neighbours' two nearest colour-0 rows, then updates colour 1 on its rows
+    // and the first halo row on each side itself (the same counters give the
+    // neighbours' decisions).  Strips of at most 512 colour-1 words.
+    uint32_t halo;
     // device workspace, offsets in 32-bit words (even where 64-bit words
     // start), all of it cleared per launch:
     //   o_err       error word
     //   o_consumed  per slot: samples the tails have finished with it
-    //   o_xhalo     (data, seq) words: [colour][strip][first, last row][wpr]
-    //   o_xfirst    (data, seq) words: [strip][first rows of levels 1..lam]
+    //   o_xhalo     (data, seq) words: [colour][strip][first, last row][wpr],
+    //               with halo [sweep parity][strip][first two, last two rows][wpr]
+    //   o_xfirst    (data, seq) words: [strip][first rows of levels 1..lam],
+    //               with halo [sample mod kStripFirstSlots][strip][...]
     //   o_ring      (data, seq) words: [slot][strip][per_wg]
     uint32_t o_err, o_consumed, o_xhalo, o_xfirst, o_ring, ws_words;
     uint32_t cnt_words;                 // 4 counts per level 0..lam
@@ -67,9 +79,9 @@ struct StripShape {
 __host__ __device__ inline uint32_t strip_row_words(uint32_t L, uint32_t l) { return (L >> l) >> 5; }
 
 // n_wg strips of R = L / n_wg >= 2 rows; lam = min(log2 R, nl) levels block
-// inside a strip.
+// inside a strip.  `halo_ok` allows the halo-row mode where the strip fits it.
 __host__ __device__ inline StripShape strip_shape(uint32_t L, uint32_t nl, uint32_t n_wg, uint32_t n_tail,
-                                                  bool measure)
+                                                  bool measure, bool halo_ok = true)
 {
     StripShape s;
     s.L = L;
@@ -81,9 +93,16 @@ __host__ __device__ inline StripShape strip_shape(uint32_t L, uint32_t nl, uint3
     s.measure = measure ? 1u : 0u;
     // enough lanes for the strip's words per colour, at most 512 when tail
     // workgroups share compute units with strips
-    const uint32_t words = s.R * s.wpr;
-    s.threads = words <= 64u ? 64u : words <= 256u ? 256u : (words <= 2048u || measure) ? 512u : 1024u;
-    s.overlap = measure && words <= s.threads ? 1u : 0u;
+    const uint32_t words = s.R * s.wpr, hwords = (s.R + 2u) * s.wpr;
+    s.halo = halo_ok && hwords <= 512u ? 1u : 0u;
+    if (s.halo) {
+        s.threads = hwords <= 64u ? 64u : hwords <= 256u ? 256u : 512u;
+        s.overlap = 0u;
+    } else {
+        s.threads = words <= 64u ? 64u : words <= 256u ? 256u : (words <= 2048u || measure) ? 512u : 1024u;
+        s.overlap = measure && words <= s.threads ? 1u : 0u;
+    }
+    const uint32_t rows = s.R + 2u + 2u * s.halo;  // rows held in LDS
     uint32_t lg = 0;
     while ((1u << (lg + 1u)) <= s.R) ++lg;
     s.lam = measure ? (lg < nl ? lg : nl) : 0u;
@@ -99,17 +118,18 @@ __host__ __device__ inline StripShape strip_shape(uint32_t L, uint32_t nl, uint3
     s.o_err = 0;
     s.o_consumed = kStripFlagStride;
     s.o_xhalo = s.o_consumed + kStripRing * kStripFlagStride;
-    s.o_xfirst = s.o_xhalo + 2u * (2u * n_wg * 2u * s.wpr);
-    s.o_ring = s.o_xfirst + 2u * n_wg * s.first_words;
+    s.o_xfirst = s.o_xhalo + 2u * (2u * n_wg * (2u + 2u * s.halo) * s.wpr);
+    s.o_ring = s.o_xfirst + 2u * (s.halo ? kStripFirstSlots : 1u) * n_wg * s.first_words;
     s.ws_words = s.o_ring + (measure ? 2u * kStripRing * n_wg * s.per_wg : 0u);
-    // strip: both colours of rows y0-1 .. y0+R, the in-strip levels and their
-    // tie coins, the pending sample's first and last rows, the lower strip's
-    // first rows, the pending counts, reduction scratch, and with overlap the
-    // copy of both colours the pending sample is measured from
+    // strip: both colours of rows y0-1 .. y0+R (y0-2 .. y0+R+1 with halo), the
+    // in-strip levels and their tie coins, the pending sample's first and last
+    // rows, the lower strip's first rows, the pending counts, reduction
+    // scratch, and with overlap or halo in measured launches the copy of both
+    // colours the pending sample is measured from
     uint32_t lv = 0;
     for (uint32_t l = 1; l <= s.lam; ++l) lv += (s.R >> l) * strip_row_words(L, l);
-    s.lds_strip = 2u * (s.R + 2u) * s.wpr + 2u * lv + 3u * s.first_words + kStripCntWords + kRedWords +
-                  (s.overlap ? 2u * (s.R + 2u) * s.wpr : 0u);
+    s.lds_strip = 2u * rows * s.wpr + 2u * lv + 3u * s.first_words + kStripCntWords + kRedWords +
+                  (s.overlap || (s.halo && measure) ? 2u * rows * s.wpr : 0u);
     // tail: operator values (int64) and M, the counts, reduction scratch, the
     // gathered level lam and the next one
     s.lds_tail = 0;

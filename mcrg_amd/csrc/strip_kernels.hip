@@ -217,7 +217,7 @@ struct StripLds {
 __device__ __forceinline__ StripLds strip_lds(const StripShape &sh, uint32_t *lds)
 {
     StripLds m;
-    m.cbs = (sh.R + 2u) * sh.wpr;
+    m.cbs = (sh.R + 2u + 2u * sh.halo) * sh.wpr;
     m.cb0 = lds;
     m.lvl = lds + 2u * m.cbs;
     uint32_t lv = 0;
@@ -559,6 +559,162 @@ __device__ void strip_body(const StripArgs &a, uint32_t *lds)
     }
 }
 
+// The flips of checkerboard word q of local row `row` (global row y) of
+// colour c, own / oth its colour's and the other colour's rows, with the
+// first two Philox calls drawn beforehand.
+__device__ __forceinline__ uint32_t strip_flip(const uint32_t *own, const uint32_t *oth, uint32_t row, uint32_t q,
+                                               uint32_t wpr, uint32_t y, uint32_t c, const Accept &acc, uint32_t k0,
+                                               uint32_t k1, uint32_t t, const u32x4 &r, const u32x4 &ref)
+{
+    const uint32_t mk = wpr - 1u, o = oth[row + q];
+    uint32_t nl, nr;
+    if (((y + c) & 1u) == 0u) {
+        nr = o;
+        nl = (o << 1) | (oth[row + ((q - 1u) & mk)] >> 31);
+    } else {
+        nl = o;
+        nr = (o >> 1) | (oth[row + ((q + 1u) & mk)] << 31);
+    }
+    return metropolis_word_drawn(own[row + q], oth[row - wpr + q], oth[row + wpr + q], nl, nr, acc, k0, k1,
+                                 y * wpr + q, t, c, r, ref);
+}
+
+// The halo-row form of strip_body (sh.halo).  Rows y0-2 .. y0+R+1 are local
+// rows 0 .. R+3; thread j updates colour-0 word j of rows 2 .. R+1 (the
+// strip's own) and colour-1 word j of rows 1 .. R+2.  One sweep t:
+//   * colour 0 on rows 2 .. R+1;
+//   * the strip's first two and last two colour-0 rows to the neighbours (one
+//     exchange per sweep, buffer t & 1).  While they travel: sample t-2 is
+//     published (the lower strip's first blocked rows of it were sent in the
+//     previous sweep), sample t-1 is measured from its copy and its first
+//     blocked rows sent, the next two updates' first Philox calls and the tie
+//     coins of sample t are drawn;
+//   * the neighbours' rows into local rows 0, 1, R+2, R+3;
+//   * colour 1 on rows 1 .. R+2.  Rows 1 and R+2 belong to the neighbours,
+//     whose decisions for them come from the same counters, so afterwards
+//     rows 1 .. R+2 hold sample t whole and the next colour-0 update needs no
+//     exchange; with measurement, the copy of sample t.
+// A sample's first blocked rows are read one sweep after they are sent, from
+// kStripFirstSlots slots per strip: a strip may send the next two samples'
+// before its upper neighbour has read them.
+template <int NT>
+__device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
+{
+    const StripShape &sh = a.sh;
+    const HbmLattice &lat = *a.lat;
+    const uint32_t L = sh.L, R = sh.R, wpr = sh.wpr, nwg = sh.n_wg, sw = lg2(wpr), mk = wpr - 1u;
+    const uint32_t w = blockIdx.x, y0 = w * R;
+    const uint32_t up = (w + nwg - 1u) & (nwg - 1u), dn = (w + 1u) & (nwg - 1u);
+    const StripLds m = strip_lds(sh, lds);
+    uint32_t *const cb0 = m.cb0, *const cb1 = m.cb0 + m.cbs;
+    const uint32_t cbs = m.cbs, fw = sh.first_words, hw = 4u * wpr;
+    uint32_t *err = a.ws + sh.o_err;
+    uint64_t *xhalo = reinterpret_cast<uint64_t *>(a.ws + sh.o_xhalo);
+    uint64_t *xfirst = reinterpret_cast<uint64_t *>(a.ws + sh.o_xfirst);
+    // rows y0-2 .. y0+R+1 from the int8 lattice
+    for (uint32_t j = threadIdx.x; j < (R + 4u) * wpr; j += NT) {
+        const uint32_t r = j >> sw, q = j & mk;
+        const uint32_t y = (y0 + r + L - 2u) & (L - 1u);
+        uint32_t d[16];
+        load64b(lat.spins + (size_t)y * L + 64u * q, d);
+        const uint32_t ce = y & 1u;  // the colour at even positions
+        cb0[ce * cbs + j] = gather_parity(d, 0u);
+        cb0[(ce ^ 1u) * cbs + j] = gather_parity(d, 1u);
+    }
+    for (uint32_t i = threadIdx.x; i < kStripRing; i += NT) m.red[i] = 0u;  // consumed counts seen
+    __syncthreads();
+    const Accept acc = lat.acc;
+    const uint32_t k0 = lat.k0, k1 = lat.k1, n = a.n_sweeps;
+    const uint32_t jw = threadIdx.x, jr = jw >> sw, q = jw & mk;
+    const bool has0 = jw < R * wpr, has1 = jw < (R + 2u) * wpr;
+    // thread jw's colour-0 word: local row 2 + jr, global row y0 + jr; its
+    // colour-1 word: local row 1 + jr, global row y0 - 1 + jr
+    const uint32_t ya = (y0 + jr) & (L - 1u), yb = (y0 + jr + L - 1u) & (L - 1u);
+    const uint32_t row0 = (2u + jr) << sw, row1 = (1u + jr) << sw;
+    u32x4 p0{0u, 0u, 0u, 0u}, p0r{0u, 0u, 0u, 0u}, p1{0u, 0u, 0u, 0u}, p1r{0u, 0u, 0u, 0u};
+    if (has0 && n > 0u) metropolis_rng(ya * wpr + q, a.t0, 0u, k0, k1, p0, p0r);
+    // neighbour words: local rows 0, 1 from the upper strip's last two rows,
+    // R+2, R+3 from the lower strip's first two (one word per thread)
+    const bool get_h = jw < hw;
+    const uint32_t h_src = jw < 2u * wpr ? up * hw + 2u * wpr + jw : dn * hw + (jw - 2u * wpr);
+    const uint32_t h_dst = (jw < 2u * wpr ? 0u : R * wpr) + jw;
+    bool ok = true;
+    for (uint32_t it = 0; it < n; ++it) {
+        const uint32_t t = a.t0 + it, seq = it + 1u, par = it & 1u;
+        if (has0) cb0[row0 + q] ^= strip_flip(cb0, cb1, row0, q, wpr, ya, 0u, acc, k0, k1, t, p0, p0r);
+        lds_barrier();
+        uint64_t *mine = xhalo + (size_t)(par * nwg + w) * hw;
+        for (uint32_t j = threadIdx.x; j < hw; j += NT)
+            ll_put(mine + j, cb0[(j < 2u * wpr ? 2u * wpr : (R - 2u) * wpr) + j], seq);
+        const uint64_t *src_h = xhalo + (size_t)par * nwg * hw + h_src;
+        uint64_t vh = get_h ? ll_peek(src_h) : 0ull;
+        if (sh.measure && it >= 2u) {
+            bool bad = false;
+            if (jw < fw)
+                m.dfirst[jw] = ll_get(xfirst + ((size_t)((it - 2u) % kStripFirstSlots) * nwg + dn) * fw + jw, it - 1u,
+                                      err, bad, a.strip_sleep);
+            ok = __syncthreads_or(bad ? 1 : 0) == 0;
+            if (ok) ok = strip_publish<NT>(a, m, it - 2u);
+            if (!ok) break;
+        }
+        if (sh.measure && it >= 1u) {
+            strip_measure<NT>(a, m, m.snap + wpr);
+            for (uint32_t i = threadIdx.x; i < fw; i += NT)
+                ll_put(xfirst + ((size_t)((it - 1u) % kStripFirstSlots) * nwg + w) * fw + i, m.frow[i], it);
+        }
+        if (has1) metropolis_rng(yb * wpr + q, t, 1u, k0, k1, p1, p1r);
+        if (has0 && it + 1u < n) metropolis_rng(ya * wpr + q, t + 1u, 0u, k0, k1, p0, p0r);
+        if (sh.measure && sh.lam > 0u) strip_coins<NT>(a, m, t);
+        bool bad = false;
+        if (get_h) {
+            if ((uint32_t)(vh >> 32) != seq) {
+                vh = ll_wait(src_h, seq, err, a.strip_sleep);
+                if ((uint32_t)(vh >> 32) != seq) bad = true;
+            }
+            cb0[h_dst] = (uint32_t)vh;
+        }
+        ok = __syncthreads_or(bad ? 1 : 0) == 0;
+        if (!ok) break;
+        if (has1) cb1[row1 + q] ^= strip_flip(cb1, cb0, row1, q, wpr, yb, 1u, acc, k0, k1, t, p1, p1r);
+        lds_barrier();
+        if (sh.measure) {
+            // sample t, measured during the next sweep's exchange
+            for (uint32_t i = threadIdx.x; i < 2u * cbs; i += NT) m.snap[i] = cb0[i];
+            lds_barrier();
+        }
+    }
+    if (ok && sh.measure && n > 0u) {
+        bool bad = false;
+        if (n >= 2u) {
+            if (jw < fw)
+                m.dfirst[jw] = ll_get(xfirst + ((size_t)((n - 2u) % kStripFirstSlots) * nwg + dn) * fw + jw, n - 1u,
+                                      err, bad, a.strip_sleep);
+            ok = __syncthreads_or(bad ? 1 : 0) == 0;
+            if (ok) ok = strip_publish<NT>(a, m, n - 2u);
+        }
+        if (ok) {
+            strip_measure<NT>(a, m, m.snap + wpr);
+            for (uint32_t i = threadIdx.x; i < fw; i += NT)
+                ll_put(xfirst + ((size_t)((n - 1u) % kStripFirstSlots) * nwg + w) * fw + i, m.frow[i], n);
+            if (jw < fw)
+                m.dfirst[jw] = ll_get(xfirst + ((size_t)((n - 1u) % kStripFirstSlots) * nwg + dn) * fw + jw, n, err,
+                                      bad, a.strip_sleep);
+            ok = __syncthreads_or(bad ? 1 : 0) == 0;
+            if (ok) ok = strip_publish<NT>(a, m, n - 1u);
+        }
+    }
+    if (!ok) return;  // the host reports the error word; the lattice is not written back
+    for (uint32_t j = threadIdx.x; j < R * wpr; j += NT) {
+        const uint32_t r = 2u + (j >> sw), qq = j & mk, y = y0 + (j >> sw);
+        const uint32_t ce = y & 1u;
+        store64b(lat.spins + (size_t)y * L + 64u * qq, cb0[ce * cbs + r * wpr + qq], cb0[(ce ^ 1u) * cbs + r * wpr + qq]);
+    }
+    if (w == 0u && threadIdx.x == 0u) {
+        lat.state[0] = a.t0 + n;
+        lat.state[1] = a.q0 + (sh.measure ? n : 0u);
+    }
+}
+
 // Reads n (data, seq) words at(i), i = 0 .. n-1, spread over the workgroup:
 // four loads in flight per thread before any is waited on; use(i, data) for
 // each.  Returns true when a wait gave up.
@@ -671,8 +827,9 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_strip(StripArgs a)
 {
     extern __shared__ uint32_t lds[];
-    if (blockIdx.x < a.sh.n_wg) strip_body<NT>(a, lds);
-    else tail_body<NT>(a, lds);
+    if (blockIdx.x >= a.sh.n_wg) tail_body<NT>(a, lds);
+    else if (a.sh.halo) strip_body_halo<NT>(a, lds);
+    else strip_body<NT>(a, lds);
 }
 
 template <int NT>

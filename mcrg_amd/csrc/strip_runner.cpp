@@ -80,12 +80,15 @@ bool StripRunner::plan(const HbmLattice &h, bool measure, StripShape &sh, int &t
         const int n = std::atoi(v);
         if (n == 1 || n == 2 || n == 4) want = (uint32_t)n;
     }
+    // MCRG_STRIP_HALO=0 keeps two exchanges per sweep where halo rows would fit
+    const char *hv = std::getenv("MCRG_STRIP_HALO");
+    const bool halo_ok = !(hv && std::atoi(hv) == 0);
     for (uint32_t n_tail : {4u, 2u, 1u}) {
         if (n_tail > want) continue;
-        sh = strip_shape(L, (uint32_t)h.n_levels, n_wg, n_tail, measure);
+        sh = strip_shape(L, (uint32_t)h.n_levels, n_wg, n_tail, measure, halo_ok);
         threads = strip_threads(sh);
         // every thread gets at most one neighbour word of each kind per exchange
-        if (2u * sh.wpr > (uint32_t)threads || sh.first_words > (uint32_t)threads) return false;
+        if ((sh.halo ? 4u : 2u) * sh.wpr > (uint32_t)threads || sh.first_words > (uint32_t)threads) return false;
         const uint32_t lds = sh.lds_words * 4u;
         if (lds > 160u * 1024u) return false;
         if ((uint32_t)strip_max_resident(threads, lds) >= sh.n_wg + sh.n_tail) return true;
