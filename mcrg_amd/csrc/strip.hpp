@@ -39,9 +39,9 @@ constexpr uint32_t kStripMaxWg = 256;
 // sent, while the sender may already send the next samples': one slot per
 // sample of the last four.
 constexpr uint32_t kStripFirstSlots = 4;
-// LDS reduction scratch: 4 totals, 4 words per wave of up to 16 waves, then
-// a wait word.
-constexpr uint32_t kRedWords = 72, kRedWait = 68;
+// LDS scratch of a strip: the consumed counts seen (kStripRing words), the
+// answer of the ring-slot wait, and the word of strip_any's votes.
+constexpr uint32_t kRedWords = 72, kRedWait = 68, kRedAny = 69;
 
 struct StripShape {
     uint32_t L, nl, n_wg, R, wpr, lam, n_tail, measure;

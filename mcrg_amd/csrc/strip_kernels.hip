@@ -202,6 +202,19 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Workgroup-wide OR of `bad` through one LDS word: a thread whose wait gave
+// up stores `tag` (never 0, different for every call of a launch), and after
+// an LDS-only barrier every thread reads the word.  Unlike __syncthreads_or
+// it neither reads the dispatch packet from memory nor waits for the wave's
+// agent-scope stores to complete.  A later call's store cannot overtake an
+// earlier call's read: no thread gets past a call whose word says failure.
+__device__ __forceinline__ bool strip_any(bool bad, uint32_t *word, uint32_t tag)
+{
+    if (bad) *word = tag;
+    lds_barrier();
+    return *word == tag;
+}
+
 __device__ __forceinline__ uint32_t lg2(uint32_t v) { return (uint32_t)__builtin_ctz(v); }
 
 // LDS of a strip workgroup, in 32-bit words: both colours of rows y0-1 ..
@@ -238,11 +251,10 @@ __device__ __forceinline__ StripLds strip_lds(const StripShape &sh, uint32_t *ld
 // consecutive coin words (the stream of tie_coin_word).  The calls of all
 // levels are spread over the workgroup at once.
 template <int NT>
-__device__ void strip_coins(const StripArgs &a, const StripLds &m, uint32_t t)
+__device__ void strip_coins(const StripArgs &a, const StripLds &m, uint32_t t, uint32_t k0, uint32_t k1)
 {
     const StripShape &sh = a.sh;
     const uint32_t L = sh.L, R = sh.R, y0 = blockIdx.x * R;
-    const uint32_t k0 = a.lat->k0, k1 = a.lat->k1;
     uint32_t calls = 0;
     for (uint32_t l = 1; l <= sh.lam; ++l) {
         const uint32_t n = (R >> l) * strip_row_words(L, l), base = (y0 >> l) * strip_row_words(L, l);
@@ -376,14 +388,22 @@ __device__ bool strip_publish(const StripArgs &a, const StripLds &m, uint32_t q)
 {
     const StripShape &sh = a.sh;
     uint32_t *err = a.ws + sh.o_err;
-    for (uint32_t l = 1; l <= sh.lam; ++l) {
-        const uint32_t wl = strip_row_words(sh.L, l), o = sh.row_off[l];
-        uint32_t d0 = 0, d1 = 0, d2 = 0;
-        for (uint32_t j = threadIdx.x; j < wl; j += NT) {
-            const uint32_t jn = (j + 1u) & (wl - 1u);
-            pair_ops(m.lrow[o + j], m.lrow[o + jn], m.dfirst[o + j], m.dfirst[o + jn], d0, d1, d2);
+    // one word of one level per thread: level l's row is words
+    // row_off[l] .. + wl of the first-rows block (levels 1..lam, halving)
+    for (uint32_t j = threadIdx.x; j < sh.first_words; j += NT) {
+        uint32_t l = 1u, o = 0u, wl = strip_row_words(sh.L, 1u);
+        while (l < sh.lam && j >= o + wl) {
+            o += wl;
+            wl >>= 1;
+            ++l;
         }
-        wave_add(m.pend + 4u * l, 3u, d0, d1, d2, 0u);
+        const uint32_t i = j - o, in = (i + 1u) & (wl - 1u);
+        uint32_t d0 = 0, d1 = 0, d2 = 0;
+        pair_ops(m.lrow[j], m.lrow[o + in], m.dfirst[j], m.dfirst[o + in], d0, d1, d2);
+        uint32_t *p = m.pend + 4u * l;
+        if (d0) atomicAdd(p, d0);
+        if (d1) atomicAdd(p + 1, d1);
+        if (d2) atomicAdd(p + 2, d2);
     }
     const uint32_t s = q % kStripRing, gen = q / kStripRing, seq = q + 1u;
     // The tail must have finished with this slot's previous sample.  The
@@ -441,9 +461,11 @@ __device__ void strip_body(const StripArgs &a, uint32_t *lds)
         cb0[(ce ^ 1u) * cbs + j] = gather_parity(d, 1u);
     }
     for (uint32_t i = threadIdx.x; i < kStripRing; i += NT) m.red[i] = 0u;  // consumed counts seen
+    if (threadIdx.x == 0) m.red[kRedAny] = 0u;
     __syncthreads();
     const Accept acc = lat.acc;
     const uint32_t k0 = lat.k0, k1 = lat.k1, mk = wpr - 1u;
+    uint32_t vote = 0;
     // With at most one word per thread (R wpr <= NT), thread j's word of
     // either colour is word y0 wpr + j, and the first two Philox calls of its
     // next update are drawn while the previous exchange is in flight.
@@ -506,7 +528,7 @@ __device__ void strip_body(const StripArgs &a, uint32_t *lds)
             if (sh.overlap && c == 0u && it > 0u) strip_measure<NT>(a, m, m.snap);
             if (one && jw < R * wpr && (c == 0u || it + 1u < a.n_sweeps))
                 metropolis_rng(my_word, c == 0u ? t : t + 1u, c ^ 1u, k0, k1, pr, pref);
-            if (c == 1u && sh.measure && sh.lam > 0u) strip_coins<NT>(a, m, t);
+            if (c == 1u && sh.measure && sh.lam > 0u) strip_coins<NT>(a, m, t, k0, k1);
             bool bad = false;
             if (get_h) {
                 if ((uint32_t)(vh >> 32) != seq) {
@@ -522,7 +544,7 @@ __device__ void strip_body(const StripArgs &a, uint32_t *lds)
                 }
                 m.dfirst[j] = (uint32_t)vf;
             }
-            ok = __syncthreads_or(bad ? 1 : 0) == 0;
+            ok = !strip_any(bad, m.red + kRedAny, ++vote);
             if (ok && c == (sh.overlap ? 1u : 0u) && sh.measure && it > 0u) ok = strip_publish<NT>(a, m, it - 1u);
         }
         if (ok && sh.measure) {
@@ -544,7 +566,7 @@ __device__ void strip_body(const StripArgs &a, uint32_t *lds)
             for (uint32_t j = threadIdx.x; j < fw; j += NT)
                 m.dfirst[j] = strip_recv_first(a, a.n_sweeps, j, err, bad);
         }
-        ok = __syncthreads_or(bad ? 1 : 0) == 0;
+        ok = !strip_any(bad, m.red + kRedAny, ++vote);
         if (ok) ok = strip_publish<NT>(a, m, a.n_sweeps - 1u);
     }
     if (!ok) return;  // the host reports the error word; the lattice is not written back
@@ -622,6 +644,7 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
         cb0[(ce ^ 1u) * cbs + j] = gather_parity(d, 1u);
     }
     for (uint32_t i = threadIdx.x; i < kStripRing; i += NT) m.red[i] = 0u;  // consumed counts seen
+    if (threadIdx.x == 0) m.red[kRedAny] = 0u;
     __syncthreads();
     const Accept acc = lat.acc;
     const uint32_t k0 = lat.k0, k1 = lat.k1, n = a.n_sweeps;
@@ -635,28 +658,41 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
     if (has0 && n > 0u) metropolis_rng(ya * wpr + q, a.t0, 0u, k0, k1, p0, p0r);
     // neighbour words: local rows 0, 1 from the upper strip's last two rows,
     // R+2, R+3 from the lower strip's first two (one word per thread)
-    const bool get_h = jw < hw;
+    const bool get_h = jw < hw, get_f = sh.measure && jw < fw;
     const uint32_t h_src = jw < 2u * wpr ? up * hw + 2u * wpr + jw : dn * hw + (jw - 2u * wpr);
     const uint32_t h_dst = (jw < 2u * wpr ? 0u : R * wpr) + jw;
+    // Device-memory loads are issued before this wave's agent-scope stores
+    // and used after them only once other work has passed: a wave's wait
+    // for a load also waits for every store it issued before the load's use.
     bool ok = true;
+    uint32_t vote = 0;
     for (uint32_t it = 0; it < n; ++it) {
         const uint32_t t = a.t0 + it, seq = it + 1u, par = it & 1u;
+        const bool pub = sh.measure && it >= 2u;
+        // the lower strip's first blocked rows of sample it-2, sent during
+        // its sweep it-1, loaded before this colour-0 update
+        const uint64_t *src_f = xfirst + ((size_t)((it + kStripFirstSlots - 2u) % kStripFirstSlots) * nwg + dn) * fw + jw;
+        uint64_t vf = pub && get_f ? ll_peek(src_f) : 0ull;
         if (has0) cb0[row0 + q] ^= strip_flip(cb0, cb1, row0, q, wpr, ya, 0u, acc, k0, k1, t, p0, p0r);
-        lds_barrier();
+        bool bad = false;
+        if (pub && get_f) {
+            if ((uint32_t)(vf >> 32) != it - 1u) {
+                vf = ll_wait(src_f, it - 1u, err, a.strip_sleep);
+                if ((uint32_t)(vf >> 32) != it - 1u) bad = true;
+            }
+            m.dfirst[jw] = (uint32_t)vf;
+        }
+        // also the barrier between the colour-0 update and the reads of its rows
+        if (strip_any(bad, m.red + kRedAny, ++vote)) {
+            ok = false;
+            break;
+        }
+        const uint64_t *src_h = xhalo + (size_t)par * nwg * hw + h_src;
+        uint64_t vh = get_h ? ll_peek(src_h) : 0ull;
         uint64_t *mine = xhalo + (size_t)(par * nwg + w) * hw;
         for (uint32_t j = threadIdx.x; j < hw; j += NT)
             ll_put(mine + j, cb0[(j < 2u * wpr ? 2u * wpr : (R - 2u) * wpr) + j], seq);
-        const uint64_t *src_h = xhalo + (size_t)par * nwg * hw + h_src;
-        uint64_t vh = get_h ? ll_peek(src_h) : 0ull;
-        if (sh.measure && it >= 2u) {
-            bool bad = false;
-            if (jw < fw)
-                m.dfirst[jw] = ll_get(xfirst + ((size_t)((it - 2u) % kStripFirstSlots) * nwg + dn) * fw + jw, it - 1u,
-                                      err, bad, a.strip_sleep);
-            ok = __syncthreads_or(bad ? 1 : 0) == 0;
-            if (ok) ok = strip_publish<NT>(a, m, it - 2u);
-            if (!ok) break;
-        }
+        if (pub && !(ok = strip_publish<NT>(a, m, it - 2u))) break;
         if (sh.measure && it >= 1u) {
             strip_measure<NT>(a, m, m.snap + wpr);
             for (uint32_t i = threadIdx.x; i < fw; i += NT)
@@ -664,8 +700,7 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
         }
         if (has1) metropolis_rng(yb * wpr + q, t, 1u, k0, k1, p1, p1r);
         if (has0 && it + 1u < n) metropolis_rng(ya * wpr + q, t + 1u, 0u, k0, k1, p0, p0r);
-        if (sh.measure && sh.lam > 0u) strip_coins<NT>(a, m, t);
-        bool bad = false;
+        if (sh.measure && sh.lam > 0u) strip_coins<NT>(a, m, t, k0, k1);
         if (get_h) {
             if ((uint32_t)(vh >> 32) != seq) {
                 vh = ll_wait(src_h, seq, err, a.strip_sleep);
@@ -673,8 +708,10 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
             }
             cb0[h_dst] = (uint32_t)vh;
         }
-        ok = __syncthreads_or(bad ? 1 : 0) == 0;
-        if (!ok) break;
+        if (strip_any(bad, m.red + kRedAny, ++vote)) {
+            ok = false;
+            break;
+        }
         if (has1) cb1[row1 + q] ^= strip_flip(cb1, cb0, row1, q, wpr, yb, 1u, acc, k0, k1, t, p1, p1r);
         lds_barrier();
         if (sh.measure) {
@@ -686,20 +723,20 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
     if (ok && sh.measure && n > 0u) {
         bool bad = false;
         if (n >= 2u) {
-            if (jw < fw)
+            if (get_f)
                 m.dfirst[jw] = ll_get(xfirst + ((size_t)((n - 2u) % kStripFirstSlots) * nwg + dn) * fw + jw, n - 1u,
                                       err, bad, a.strip_sleep);
-            ok = __syncthreads_or(bad ? 1 : 0) == 0;
+            ok = !strip_any(bad, m.red + kRedAny, ++vote);
             if (ok) ok = strip_publish<NT>(a, m, n - 2u);
         }
         if (ok) {
             strip_measure<NT>(a, m, m.snap + wpr);
             for (uint32_t i = threadIdx.x; i < fw; i += NT)
                 ll_put(xfirst + ((size_t)((n - 1u) % kStripFirstSlots) * nwg + w) * fw + i, m.frow[i], n);
-            if (jw < fw)
+            if (get_f)
                 m.dfirst[jw] = ll_get(xfirst + ((size_t)((n - 1u) % kStripFirstSlots) * nwg + dn) * fw + jw, n, err,
                                       bad, a.strip_sleep);
-            ok = __syncthreads_or(bad ? 1 : 0) == 0;
+            ok = !strip_any(bad, m.red + kRedAny, ++vote);
             if (ok) ok = strip_publish<NT>(a, m, n - 1u);
         }
     }
