@@ -106,6 +106,7 @@ __device__ __forceinline__ uint32_t wave_sum0(uint32_t v)
 
 __device__ __forceinline__ void wave_add(uint32_t *dst, uint32_t n, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3)
 {
+    if (__ballot((c0 | c1 | c2 | c3) != 0u) == 0ull) return;  // the wave has nothing to add
     const uint32_t v[4] = {wave_sum0(c0), wave_sum0(c1), n > 2u ? wave_sum0(c2) : 0u, n > 3u ? wave_sum0(c3) : 0u};
     if ((threadIdx.x & 63u) == 0u)
         for (uint32_t k = 0; k < n; ++k)
@@ -260,7 +261,9 @@ __device__ void strip_coins(const StripArgs &a, const StripLds &m, uint32_t t, u
         const uint32_t n = (R >> l) * strip_row_words(L, l), base = (y0 >> l) * strip_row_words(L, l);
         calls += ((base + n - 1u) >> 2) - (base >> 2) + 1u;
     }
-    for (uint32_t j = threadIdx.x; j < calls; j += NT) {
+    // on the workgroup's last threads, so that the first ones' Philox calls
+    // for the next update run beside them
+    for (uint32_t j = NT - 1u - threadIdx.x; j < calls; j += NT) {
         uint32_t l = 1, g = j, off = 0, n = 0, base = 0;
         for (;; ++l) {
             n = (R >> l) * strip_row_words(L, l);
@@ -311,7 +314,9 @@ __device__ void strip_levels(const StripArgs &a, const StripLds &m, const uint32
             const uint32_t *ar = cur + (j - q), *dr = ar + wl;
             pair_ops(ar[q], ar[qn], dr[q], dr[qn], d0, d1, d2);
         }
-        for (uint32_t j = threadIdx.x; j < wl; j += NT) {
+        // the first and last rows and the next blocking on other threads than
+        // the pair counts, so that their latencies overlap
+        for (uint32_t j = (threadIdx.x + NT / 2u) % NT; j < wl; j += NT) {
             m.frow[sh.row_off[l] + j] = cur[j];
             m.lrow[sh.row_off[l] + j] = cur[((Rl - 1u) << sl) + j];
         }
@@ -320,7 +325,7 @@ __device__ void strip_levels(const StripArgs &a, const StripLds &m, const uint32
             uint32_t *nxt = cur + (Rl << sl);
             coin += Rl << sl;  // level l + 1's coins follow level l's
             const uint32_t wo = wl >> 1, so = sl - 1u;
-            for (uint32_t j = threadIdx.x; j < (Rl >> 1) << so; j += NT) {
+            for (uint32_t j = NT - 1u - threadIdx.x; j < (Rl >> 1) << so; j += NT) {
                 const uint32_t jo = j >> so, qo = j & (wo - 1u);
                 const uint32_t c = coin[j];
                 uint32_t o = 0;
@@ -349,10 +354,8 @@ __device__ void strip_measure(const StripArgs &a, const StripLds &m, const uint3
 {
     const StripShape &sh = a.sh;
     const uint32_t R = sh.R, wpr = sh.wpr, y0 = blockIdx.x * R, sw = lg2(wpr), sc = lg2(R * wpr);
-    // pend was last read by the previous sample's publication, which ended
-    // with a barrier
-    for (uint32_t i = threadIdx.x; i < sh.cnt_words; i += NT) m.pend[i] = 0u;
-    lds_barrier();
+    // pend is zero: cleared at the start of the launch and by the previous
+    // sample's publication, which ended with a barrier
     uint32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0;
     for (uint32_t j = threadIdx.x; j < 2u * R * wpr; j += NT) {
         const uint32_t c = j >> sc, jj = j & ((1u << sc) - 1u);
@@ -390,7 +393,7 @@ __device__ bool strip_publish(const StripArgs &a, const StripLds &m, uint32_t q)
     uint32_t *err = a.ws + sh.o_err;
     // one word of one level per thread: level l's row is words
     // row_off[l] .. + wl of the first-rows block (levels 1..lam, halving)
-    for (uint32_t j = threadIdx.x; j < sh.first_words; j += NT) {
+    for (uint32_t j = NT - 1u - threadIdx.x; j < sh.first_words; j += NT) {
         uint32_t l = 1u, o = 0u, wl = strip_row_words(sh.L, 1u);
         while (l < sh.lam && j >= o + wl) {
             o += wl;
@@ -429,8 +432,14 @@ __device__ bool strip_publish(const StripArgs &a, const StripLds &m, uint32_t q)
     lds_barrier();  // every wave's counts are in pend, and the wait's answer
     if (gen > 0u && m.red[kRedWait] == 0u) return false;
     uint64_t *out = reinterpret_cast<uint64_t *>(a.ws + sh.o_ring) + (size_t)(s * sh.n_wg + blockIdx.x) * sh.per_wg;
-    for (uint32_t j = threadIdx.x; j < sh.per_wg; j += NT)
-        ll_put(out + j, j < sh.cnt_words ? m.pend[j] : m.frow[sh.row_off[sh.lam] + (j - sh.cnt_words)], seq);
+    for (uint32_t j = threadIdx.x; j < sh.per_wg; j += NT) {
+        if (j < sh.cnt_words) {
+            ll_put(out + j, m.pend[j], seq);
+            m.pend[j] = 0u;  // for the next sample (each word is read by this thread only)
+        } else {
+            ll_put(out + j, m.frow[sh.row_off[sh.lam] + (j - sh.cnt_words)], seq);
+        }
+    }
     lds_barrier();  // pend and frow are rewritten by the next measurement
     return true;
 }
@@ -461,6 +470,7 @@ __device__ void strip_body(const StripArgs &a, uint32_t *lds)
         cb0[(ce ^ 1u) * cbs + j] = gather_parity(d, 1u);
     }
     for (uint32_t i = threadIdx.x; i < kStripRing; i += NT) m.red[i] = 0u;  // consumed counts seen
+    for (uint32_t i = threadIdx.x; i < kStripCntWords; i += NT) m.pend[i] = 0u;
     if (threadIdx.x == 0) m.red[kRedAny] = 0u;
     __syncthreads();
     const Accept acc = lat.acc;
@@ -644,6 +654,7 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
         cb0[(ce ^ 1u) * cbs + j] = gather_parity(d, 1u);
     }
     for (uint32_t i = threadIdx.x; i < kStripRing; i += NT) m.red[i] = 0u;  // consumed counts seen
+    for (uint32_t i = threadIdx.x; i < kStripCntWords; i += NT) m.pend[i] = 0u;
     if (threadIdx.x == 0) m.red[kRedAny] = 0u;
     __syncthreads();
     const Accept acc = lat.acc;
