@@ -340,7 +340,7 @@ mcrg_status mcrg_chain_sweep_static(mcrg_chain ch, int32_t n_sweeps, int32_t mea
                   "creation)");
         return MCRG_ERR_DEVICE;
     }
-    const size_t need = ch->strip.workspace_words(ch->host, measure != 0);
+    const size_t need = ch->strip.workspace_words(ch->host, measure != 0, kStaticScratchWords);
     MCRG_REQUIRE(need > 0, "mcrg_chain_sweep_static runs chains on the strip schedule only: the Metropolis sampler, "
                            "64 <= L <= 8192 (L = 16384 for unmeasured sweeps)");
     MCRG_REQUIRE(need <= kStaticScratchWords, "this chain needs more workspace than the static array holds");

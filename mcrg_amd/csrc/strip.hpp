@@ -35,10 +35,13 @@ constexpr uint32_t kStripCntWords = 64;    // per-slot operator counts, 4 per le
 // A wait gives up (and sets the error word) after 2 s of the 100 MHz clock.
 constexpr uint64_t kStripWaitTicks = 200000000ull;
 constexpr uint32_t kStripMaxWg = 256;
-// With halo rows a sample's first rows are read two sweeps after they are
-// sent, while the sender may already send the next samples': one slot per
-// sample of the last four.
-constexpr uint32_t kStripFirstSlots = 4;
+// Slots per strip of a sample's first blocked rows with halo rows: they are
+// read a sweep after they are sent, while the sender may already send the
+// next samples' (up to two more; up to kStripSnapSlots + 2 more when
+// measuring workgroups run behind their strips).
+constexpr uint32_t kStripFirstSlots = 4, kStripFirstSlotsSplit = 8;
+// Sample copies a strip may publish ahead of its measuring workgroup.
+constexpr uint32_t kStripSnapSlots = 4;
 // LDS scratch of a strip: the consumed counts seen (kStripRing words), the
 // answer of the ring-slot wait, and the word of strip_any's votes.
 constexpr uint32_t kRedWords = 72, kRedWait = 68, kRedAny = 69;
@@ -56,6 +59,11 @@ struct StripShape {
     // and the first halo row on each side itself (the same counters give the
     // neighbours' decisions).  Strips of at most 512 colour-1 words.
     uint32_t halo;
+    // measured halo launches with a second workgroup per strip (split): the
+    // strip publishes each sample's rows y0-1 .. y0+R (snap_words (data, seq)
+    // words) and workgroup n_wg + w measures them, so a strip's sweeps no
+    // longer wait for its measurements
+    uint32_t split, n_meas, snap_words, first_slots;
     // device workspace, offsets in 32-bit words (even where 64-bit words
     // start), all of it cleared per launch:
     //   o_err       error word
@@ -63,9 +71,12 @@ struct StripShape {
     //   o_xhalo     (data, seq) words: [colour][strip][first, last row][wpr],
     //               with halo [sweep parity][strip][first two, last two rows][wpr]
     //   o_xfirst    (data, seq) words: [strip][first rows of levels 1..lam],
-    //               with halo [sample mod kStripFirstSlots][strip][...]
+    //               with halo [sample mod first_slots][strip][...]
     //   o_ring      (data, seq) words: [slot][strip][per_wg]
-    uint32_t o_err, o_consumed, o_xhalo, o_xfirst, o_ring, ws_words;
+    //   o_snapc     with split, per strip: samples its measuring workgroup
+    //               has copied (kStripFlagStride apart)
+    //   o_snap      with split, (data, seq) words: [slot][strip][snap_words]
+    uint32_t o_err, o_consumed, o_xhalo, o_xfirst, o_ring, o_snapc, o_snap, ws_words;
     uint32_t cnt_words;                 // 4 counts per level 0..lam
     uint32_t first_words;               // one row of each level 1..lam
     uint32_t row_off[kMaxLevels + 1];   // level l's row within those
@@ -79,9 +90,10 @@ struct StripShape {
 __host__ __device__ inline uint32_t strip_row_words(uint32_t L, uint32_t l) { return (L >> l) >> 5; }
 
 // n_wg strips of R = L / n_wg >= 2 rows; lam = min(log2 R, nl) levels block
-// inside a strip.  `halo_ok` allows the halo-row mode where the strip fits it.
+// inside a strip.  `halo_ok` allows the halo-row mode where the strip fits it,
+// `split_ok` its measuring workgroups.
 __host__ __device__ inline StripShape strip_shape(uint32_t L, uint32_t nl, uint32_t n_wg, uint32_t n_tail,
-                                                  bool measure, bool halo_ok = true)
+                                                  bool measure, bool halo_ok = true, bool split_ok = false)
 {
     StripShape s;
     s.L = L;
@@ -95,14 +107,25 @@ __host__ __device__ inline StripShape strip_shape(uint32_t L, uint32_t nl, uint3
     // workgroups share compute units with strips
     const uint32_t words = s.R * s.wpr, hwords = (s.R + 2u) * s.wpr;
     s.halo = halo_ok && hwords <= 512u ? 1u : 0u;
-    if (s.halo) {
+    if (s.halo && measure && split_ok && words <= 256u && hwords <= 512u) {
+        // measuring workgroups: 2 n_wg + n_tail workgroups resident at once,
+        // three on some compute units, of at most four waves; threads j <
+        // hwords - 256 update a second colour-1 word
+        s.threads = hwords <= 64u ? 64u : 256u;
+        s.overlap = 0u;
+    } else if (s.halo) {
         s.threads = hwords <= 64u ? 64u : hwords <= 256u ? 256u : 512u;
         s.overlap = 0u;
+        split_ok = false;
     } else {
         s.threads = words <= 64u ? 64u : words <= 256u ? 256u : (words <= 2048u || measure) ? 512u : 1024u;
         s.overlap = measure && words <= s.threads ? 1u : 0u;
     }
     const uint32_t rows = s.R + 2u + 2u * s.halo;  // rows held in LDS
+    s.split = s.halo && measure && split_ok ? 1u : 0u;
+    s.n_meas = s.split ? n_wg : 0u;
+    s.snap_words = s.split ? 2u * (s.R + 2u) * s.wpr : 0u;
+    s.first_slots = s.halo ? (s.split ? kStripFirstSlotsSplit : kStripFirstSlots) : 1u;
     uint32_t lg = 0;
     while ((1u << (lg + 1u)) <= s.R) ++lg;
     s.lam = measure ? (lg < nl ? lg : nl) : 0u;
@@ -119,8 +142,10 @@ __host__ __device__ inline StripShape strip_shape(uint32_t L, uint32_t nl, uint3
     s.o_consumed = kStripFlagStride;
     s.o_xhalo = s.o_consumed + kStripRing * kStripFlagStride;
     s.o_xfirst = s.o_xhalo + 2u * (2u * n_wg * (2u + 2u * s.halo) * s.wpr);
-    s.o_ring = s.o_xfirst + 2u * (s.halo ? kStripFirstSlots : 1u) * n_wg * s.first_words;
-    s.ws_words = s.o_ring + (measure ? 2u * kStripRing * n_wg * s.per_wg : 0u);
+    s.o_ring = s.o_xfirst + 2u * s.first_slots * n_wg * s.first_words;
+    s.o_snapc = s.o_ring + (measure ? 2u * kStripRing * n_wg * s.per_wg : 0u);
+    s.o_snap = s.o_snapc + (s.split ? n_wg * kStripFlagStride : 0u);
+    s.ws_words = s.o_snap + 2u * kStripSnapSlots * n_wg * s.snap_words;
     // strip: both colours of rows y0-1 .. y0+R (y0-2 .. y0+R+1 with halo), the
     // in-strip levels and their tie coins, the pending sample's first and last
     // rows, the lower strip's first rows, the pending counts, reduction
@@ -192,8 +217,9 @@ class StripRunner {
     StripRunner &operator=(const StripRunner &) = delete;
     ~StripRunner();
     bool usable(const HbmLattice &h, bool measure);
-    // Workspace words a launch of this shape needs (0 when it cannot run here).
-    size_t workspace_words(const HbmLattice &h, bool measure);
+    // Workspace words a launch of this shape needs (0 when it cannot run
+    // here); with max_words > 0, of a launch that fits in that many.
+    size_t workspace_words(const HbmLattice &h, bool measure, size_t max_words = 0);
     // With fixed_ws the launch uses that workspace of fixed_words words and
     // allocates nothing (hipErrorInvalidValue when it needs more).
     hipError_t run(const HbmLattice *d, const HbmLattice &h, uint32_t t0, uint32_t q0, uint32_t n, bool measure,
@@ -203,7 +229,9 @@ class StripRunner {
     hipError_t failed(hipStream_t s, bool &out);
 
   private:
-    bool plan(const HbmLattice &h, bool measure, StripShape &sh, int &threads);
+    // The launch shape; max_words > 0 drops the measuring workgroups when
+    // their workspace would not fit in that many words.
+    bool plan(const HbmLattice &h, bool measure, StripShape &sh, int &threads, size_t max_words);
     uint32_t *ws_ = nullptr;
     size_t ws_words_ = 0;
     uint32_t *last_ws_ = nullptr;  // workspace of the last launch (its error word)

@@ -226,11 +226,13 @@ __device__ __forceinline__ uint32_t lg2(uint32_t v) { return (uint32_t)__builtin
 struct StripLds {
     uint32_t *cb0, *lvl, *coin, *frow, *lrow, *dfirst, *pend, *red, *snap;
     uint32_t cbs;
+    uint32_t w;  // the strip
 };
 
-__device__ __forceinline__ StripLds strip_lds(const StripShape &sh, uint32_t *lds)
+__device__ __forceinline__ StripLds strip_lds(const StripShape &sh, uint32_t *lds, uint32_t w)
 {
     StripLds m;
+    m.w = w;
     m.cbs = (sh.R + 2u + 2u * sh.halo) * sh.wpr;
     m.cb0 = lds;
     m.lvl = lds + 2u * m.cbs;
@@ -255,7 +257,7 @@ template <int NT>
 __device__ void strip_coins(const StripArgs &a, const StripLds &m, uint32_t t, uint32_t k0, uint32_t k1)
 {
     const StripShape &sh = a.sh;
-    const uint32_t L = sh.L, R = sh.R, y0 = blockIdx.x * R;
+    const uint32_t L = sh.L, R = sh.R, y0 = m.w * R;
     uint32_t calls = 0;
     for (uint32_t l = 1; l <= sh.lam; ++l) {
         const uint32_t n = (R >> l) * strip_row_words(L, l), base = (y0 >> l) * strip_row_words(L, l);
@@ -353,7 +355,7 @@ template <int NT>
 __device__ void strip_measure(const StripArgs &a, const StripLds &m, const uint32_t *src)
 {
     const StripShape &sh = a.sh;
-    const uint32_t R = sh.R, wpr = sh.wpr, y0 = blockIdx.x * R, sw = lg2(wpr), sc = lg2(R * wpr);
+    const uint32_t R = sh.R, wpr = sh.wpr, y0 = m.w * R, sw = lg2(wpr), sc = lg2(R * wpr);
     // pend is zero: cleared at the start of the launch and by the previous
     // sample's publication, which ended with a barrier
     uint32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0;
@@ -431,7 +433,7 @@ __device__ bool strip_publish(const StripArgs &a, const StripLds &m, uint32_t q)
     }
     lds_barrier();  // every wave's counts are in pend, and the wait's answer
     if (gen > 0u && m.red[kRedWait] == 0u) return false;
-    uint64_t *out = reinterpret_cast<uint64_t *>(a.ws + sh.o_ring) + (size_t)(s * sh.n_wg + blockIdx.x) * sh.per_wg;
+    uint64_t *out = reinterpret_cast<uint64_t *>(a.ws + sh.o_ring) + (size_t)(s * sh.n_wg + m.w) * sh.per_wg;
     for (uint32_t j = threadIdx.x; j < sh.per_wg; j += NT) {
         if (j < sh.cnt_words) {
             ll_put(out + j, m.pend[j], seq);
@@ -452,7 +454,7 @@ __device__ void strip_body(const StripArgs &a, uint32_t *lds)
     const uint32_t L = sh.L, R = sh.R, wpr = sh.wpr, nwg = sh.n_wg, sw = lg2(wpr);
     const uint32_t w = blockIdx.x, y0 = w * R;
     const uint32_t up = (w + nwg - 1u) & (nwg - 1u), dn = (w + 1u) & (nwg - 1u);
-    const StripLds m = strip_lds(sh, lds);
+    const StripLds m = strip_lds(sh, lds, w);
     uint32_t *const cb0 = m.cb0;
     const uint32_t cbs = m.cbs, fw = sh.first_words;
     uint32_t *ws = a.ws;
@@ -613,22 +615,25 @@ __device__ __forceinline__ uint32_t strip_flip(const uint32_t *own, const uint32
 
 // The halo-row form of strip_body (sh.halo).  Rows y0-2 .. y0+R+1 are local
 // rows 0 .. R+3; thread j updates colour-0 word j of rows 2 .. R+1 (the
-// strip's own) and colour-1 word j of rows 1 .. R+2.  One sweep t:
+// strip's own) and colour-1 words j and j + NT of rows 1 .. R+2.  One sweep t:
 //   * colour 0 on rows 2 .. R+1;
 //   * the strip's first two and last two colour-0 rows to the neighbours (one
-//     exchange per sweep, buffer t & 1).  While they travel: sample t-2 is
-//     published (the lower strip's first blocked rows of it were sent in the
-//     previous sweep), sample t-1 is measured from its copy and its first
-//     blocked rows sent, the next two updates' first Philox calls and the tie
-//     coins of sample t are drawn;
+//     exchange per sweep, buffer t & 1).  While they travel, when the strip
+//     measures its own samples: sample t-2 is published (the lower strip's
+//     first blocked rows of it were sent in the previous sweep), sample t-1 is
+//     measured from its copy and its first blocked rows sent, the tie coins of
+//     sample t are drawn; and the next two updates' first Philox calls;
 //   * the neighbours' rows into local rows 0, 1, R+2, R+3;
 //   * colour 1 on rows 1 .. R+2.  Rows 1 and R+2 belong to the neighbours,
 //     whose decisions for them come from the same counters, so afterwards
 //     rows 1 .. R+2 hold sample t whole and the next colour-0 update needs no
-//     exchange; with measurement, the copy of sample t.
+//     exchange;
+//   * when measuring, the copy of sample t: in LDS, or with split to the
+//     strip's measuring workgroup (slot t mod kStripSnapSlots, once that
+//     workgroup has copied the slot's previous sample).
 // A sample's first blocked rows are read one sweep after they are sent, from
-// kStripFirstSlots slots per strip: a strip may send the next two samples'
-// before its upper neighbour has read them.
+// first_slots slots per strip: a strip may send the next two samples' before
+// its upper neighbour has read them.
 template <int NT>
 __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
 {
@@ -637,9 +642,9 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
     const uint32_t L = sh.L, R = sh.R, wpr = sh.wpr, nwg = sh.n_wg, sw = lg2(wpr), mk = wpr - 1u;
     const uint32_t w = blockIdx.x, y0 = w * R;
     const uint32_t up = (w + nwg - 1u) & (nwg - 1u), dn = (w + 1u) & (nwg - 1u);
-    const StripLds m = strip_lds(sh, lds);
+    const StripLds m = strip_lds(sh, lds, w);
     uint32_t *const cb0 = m.cb0, *const cb1 = m.cb0 + m.cbs;
-    const uint32_t cbs = m.cbs, fw = sh.first_words, hw = 4u * wpr;
+    const uint32_t cbs = m.cbs, fw = sh.first_words, hw = 4u * wpr, fs = sh.first_slots;
     uint32_t *err = a.ws + sh.o_err;
     uint64_t *xhalo = reinterpret_cast<uint64_t *>(a.ws + sh.o_xhalo);
     uint64_t *xfirst = reinterpret_cast<uint64_t *>(a.ws + sh.o_xfirst);
@@ -659,33 +664,44 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
     __syncthreads();
     const Accept acc = lat.acc;
     const uint32_t k0 = lat.k0, k1 = lat.k1, n = a.n_sweeps;
+    const bool own = sh.measure && !sh.split;  // this workgroup measures its samples
     const uint32_t jw = threadIdx.x, jr = jw >> sw, q = jw & mk;
-    const bool has0 = jw < R * wpr, has1 = jw < (R + 2u) * wpr;
+    const uint32_t j2 = jw + NT, jr2 = j2 >> sw, q2 = j2 & mk;
+    const bool has0 = jw < R * wpr, has1 = jw < (R + 2u) * wpr, has2 = j2 < (R + 2u) * wpr;
     // thread jw's colour-0 word: local row 2 + jr, global row y0 + jr; its
-    // colour-1 word: local row 1 + jr, global row y0 - 1 + jr
+    // colour-1 words: local row 1 + jr, global row y0 - 1 + jr, and (has2)
+    // local row 1 + jr2, global row y0 - 1 + jr2
     const uint32_t ya = (y0 + jr) & (L - 1u), yb = (y0 + jr + L - 1u) & (L - 1u);
-    const uint32_t row0 = (2u + jr) << sw, row1 = (1u + jr) << sw;
+    const uint32_t yc = (y0 + jr2 + L - 1u) & (L - 1u);
+    const uint32_t row0 = (2u + jr) << sw, row1 = (1u + jr) << sw, row2 = (1u + jr2) << sw;
     u32x4 p0{0u, 0u, 0u, 0u}, p0r{0u, 0u, 0u, 0u}, p1{0u, 0u, 0u, 0u}, p1r{0u, 0u, 0u, 0u};
     if (has0 && n > 0u) metropolis_rng(ya * wpr + q, a.t0, 0u, k0, k1, p0, p0r);
     // neighbour words: local rows 0, 1 from the upper strip's last two rows,
     // R+2, R+3 from the lower strip's first two (one word per thread)
-    const bool get_h = jw < hw, get_f = sh.measure && jw < fw;
+    const bool get_h = jw < hw, get_f = own && jw < fw;
     const uint32_t h_src = jw < 2u * wpr ? up * hw + 2u * wpr + jw : dn * hw + (jw - 2u * wpr);
     const uint32_t h_dst = (jw < 2u * wpr ? 0u : R * wpr) + jw;
+    // with split: the samples the measuring workgroup has copied, as last seen
+    const uint32_t *snapc = a.ws + sh.o_snapc + w * kStripFlagStride;
+    uint64_t *snap = reinterpret_cast<uint64_t *>(a.ws + sh.o_snap);
+    uint32_t copied = 0;
     // Device-memory loads are issued before this wave's agent-scope stores
     // and used after them only once other work has passed: a wave's wait
     // for a load also waits for every store it issued before the load's use.
-    bool ok = true;
+    bool ok = true, bad = false;
     uint32_t vote = 0;
     for (uint32_t it = 0; it < n; ++it) {
         const uint32_t t = a.t0 + it, seq = it + 1u, par = it & 1u;
-        const bool pub = sh.measure && it >= 2u;
+        const bool pub = own && it >= 2u;
         // the lower strip's first blocked rows of sample it-2, sent during
         // its sweep it-1, loaded before this colour-0 update
-        const uint64_t *src_f = xfirst + ((size_t)((it + kStripFirstSlots - 2u) % kStripFirstSlots) * nwg + dn) * fw + jw;
+        const uint64_t *src_f = xfirst + ((size_t)((it + fs - 2u) % fs) * nwg + dn) * fw + jw;
         uint64_t vf = pub && get_f ? ll_peek(src_f) : 0ull;
+        // with split, slot it mod kStripSnapSlots is free once sample
+        // it - kStripSnapSlots has been copied
+        const uint32_t need = it >= kStripSnapSlots ? it - kStripSnapSlots + 1u : 0u;
+        const uint32_t copied_now = sh.split && copied < need ? ld_relaxed(snapc) : copied;
         if (has0) cb0[row0 + q] ^= strip_flip(cb0, cb1, row0, q, wpr, ya, 0u, acc, k0, k1, t, p0, p0r);
-        bool bad = false;
         if (pub && get_f) {
             if ((uint32_t)(vf >> 32) != it - 1u) {
                 vf = ll_wait(src_f, it - 1u, err, a.strip_sleep);
@@ -704,14 +720,14 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
         for (uint32_t j = threadIdx.x; j < hw; j += NT)
             ll_put(mine + j, cb0[(j < 2u * wpr ? 2u * wpr : (R - 2u) * wpr) + j], seq);
         if (pub && !(ok = strip_publish<NT>(a, m, it - 2u))) break;
-        if (sh.measure && it >= 1u) {
+        if (own && it >= 1u) {
             strip_measure<NT>(a, m, m.snap + wpr);
             for (uint32_t i = threadIdx.x; i < fw; i += NT)
-                ll_put(xfirst + ((size_t)((it - 1u) % kStripFirstSlots) * nwg + w) * fw + i, m.frow[i], it);
+                ll_put(xfirst + ((size_t)((it - 1u) % fs) * nwg + w) * fw + i, m.frow[i], it);
         }
         if (has1) metropolis_rng(yb * wpr + q, t, 1u, k0, k1, p1, p1r);
         if (has0 && it + 1u < n) metropolis_rng(ya * wpr + q, t + 1u, 0u, k0, k1, p0, p0r);
-        if (sh.measure && sh.lam > 0u) strip_coins<NT>(a, m, t, k0, k1);
+        if (own && sh.lam > 0u) strip_coins<NT>(a, m, t, k0, k1);
         if (get_h) {
             if ((uint32_t)(vh >> 32) != seq) {
                 vh = ll_wait(src_h, seq, err, a.strip_sleep);
@@ -724,29 +740,53 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
             break;
         }
         if (has1) cb1[row1 + q] ^= strip_flip(cb1, cb0, row1, q, wpr, yb, 1u, acc, k0, k1, t, p1, p1r);
+        if (NT == 256 && has2) {
+            // a second word (measuring-workgroup launches only): its Philox
+            // calls drawn here
+            u32x4 p2, p2r;
+            metropolis_rng(yc * wpr + q2, t, 1u, k0, k1, p2, p2r);
+            cb1[row2 + q2] ^= strip_flip(cb1, cb0, row2, q2, wpr, yc, 1u, acc, k0, k1, t, p2, p2r);
+        }
         lds_barrier();
-        if (sh.measure) {
+        if (own) {
             // sample t, measured during the next sweep's exchange
             for (uint32_t i = threadIdx.x; i < 2u * cbs; i += NT) m.snap[i] = cb0[i];
             lds_barrier();
+        } else if (sh.split) {
+            // sample t to the measuring workgroup: both colours of local rows
+            // 1 .. R+2; a failed wait is voted on at the next barrier
+            copied = copied_now;
+            if (copied < need) {
+                copied = flag_wait(snapc, need, err) ? need : 0u;
+                bad = copied < need;
+            }
+            if (!bad) {
+                uint64_t *dst = snap + ((size_t)(it % kStripSnapSlots) * nwg + w) * sh.snap_words;
+                const uint32_t half = sh.snap_words >> 1;
+                for (uint32_t i = threadIdx.x; i < sh.snap_words; i += NT) {
+                    const uint32_t c = i >= half ? 1u : 0u;
+                    ll_put(dst + i, cb0[c * cbs + wpr + (i - c * half)], seq);
+                }
+            }
+            lds_barrier();  // the copy is read before the next colour-0 update
         }
     }
-    if (ok && sh.measure && n > 0u) {
-        bool bad = false;
+    if (ok && sh.split && strip_any(bad, m.red + kRedAny, ++vote)) ok = false;
+    if (ok && own && n > 0u) {
         if (n >= 2u) {
             if (get_f)
-                m.dfirst[jw] = ll_get(xfirst + ((size_t)((n - 2u) % kStripFirstSlots) * nwg + dn) * fw + jw, n - 1u,
-                                      err, bad, a.strip_sleep);
+                m.dfirst[jw] = ll_get(xfirst + ((size_t)((n - 2u) % fs) * nwg + dn) * fw + jw, n - 1u, err, bad,
+                                      a.strip_sleep);
             ok = !strip_any(bad, m.red + kRedAny, ++vote);
             if (ok) ok = strip_publish<NT>(a, m, n - 2u);
         }
         if (ok) {
             strip_measure<NT>(a, m, m.snap + wpr);
             for (uint32_t i = threadIdx.x; i < fw; i += NT)
-                ll_put(xfirst + ((size_t)((n - 1u) % kStripFirstSlots) * nwg + w) * fw + i, m.frow[i], n);
+                ll_put(xfirst + ((size_t)((n - 1u) % fs) * nwg + w) * fw + i, m.frow[i], n);
             if (get_f)
-                m.dfirst[jw] = ll_get(xfirst + ((size_t)((n - 1u) % kStripFirstSlots) * nwg + dn) * fw + jw, n, err,
-                                      bad, a.strip_sleep);
+                m.dfirst[jw] = ll_get(xfirst + ((size_t)((n - 1u) % fs) * nwg + dn) * fw + jw, n, err, bad,
+                                      a.strip_sleep);
             ok = !strip_any(bad, m.red + kRedAny, ++vote);
             if (ok) ok = strip_publish<NT>(a, m, n - 1u);
         }
@@ -788,6 +828,74 @@ __device__ bool ll_gather(uint32_t n, uint32_t seq, uint32_t *err, uint32_t slee
     return bad;
 }
 
+// The measuring workgroup n_wg + w of a split launch: strip w's samples, one
+// per sweep, from the copies the strip publishes (rows y0-1 .. y0+R of both
+// colours, into local rows 1 .. R+2 of the halo layout).  Per sample k: the
+// tie coins while the copy arrives; the copy, then its slot released to the
+// strip; the publication of sample k-1 (the lower measuring workgroup's first
+// blocked rows of it were sent one sample ago); the measurement of k and its
+// first blocked rows to the upper measuring workgroup.
+template <int NT>
+__device__ void strip_body_meas(const StripArgs &a, uint32_t *lds)
+{
+    const StripShape &sh = a.sh;
+    const HbmLattice &lat = *a.lat;
+    const uint32_t R = sh.R, wpr = sh.wpr, nwg = sh.n_wg;
+    const uint32_t w = blockIdx.x - nwg, dn = (w + 1u) & (nwg - 1u);
+    const StripLds m = strip_lds(sh, lds, w);
+    const uint32_t cbs = m.cbs, fw = sh.first_words, fs = sh.first_slots, n = a.n_sweeps;
+    const uint32_t half = sh.snap_words >> 1;
+    uint32_t *err = a.ws + sh.o_err;
+    uint64_t *xfirst = reinterpret_cast<uint64_t *>(a.ws + sh.o_xfirst);
+    const uint64_t *snap = reinterpret_cast<const uint64_t *>(a.ws + sh.o_snap);
+    uint32_t *snapc = a.ws + sh.o_snapc + w * kStripFlagStride;
+    for (uint32_t i = threadIdx.x; i < kStripRing; i += NT) m.red[i] = 0u;  // consumed counts seen
+    for (uint32_t i = threadIdx.x; i < kStripCntWords; i += NT) m.pend[i] = 0u;
+    if (threadIdx.x == 0) m.red[kRedAny] = 0u;
+    __syncthreads();
+    const uint32_t k0 = lat.k0, k1 = lat.k1, jw = threadIdx.x;
+    const bool get_f = jw < fw;
+    bool ok = true;
+    uint32_t vote = 0;
+    for (uint32_t k = 0; k < n && ok; ++k) {
+        // loads first: sample k-1's first rows from below, sample k's copy
+        const uint64_t *src_f = xfirst + ((size_t)((k + fs - 1u) % fs) * nwg + dn) * fw + jw;
+        uint64_t vf = k >= 1u && get_f ? ll_peek(src_f) : 0ull;
+        const uint64_t *slot = snap + ((size_t)(k % kStripSnapSlots) * nwg + w) * sh.snap_words;
+        if (sh.lam > 0u) strip_coins<NT>(a, m, a.t0 + k, k0, k1);
+        bool bad = ll_gather<NT>(
+            sh.snap_words, k + 1u, err, a.strip_sleep, [&](uint32_t i) { return slot + i; },
+            [&](uint32_t i, uint32_t v) {
+                const uint32_t c = i >= half ? 1u : 0u;
+                m.cb0[c * cbs + wpr + (i - c * half)] = v;
+            });
+        if (k >= 1u && get_f) {
+            if ((uint32_t)(vf >> 32) != k) {
+                vf = ll_wait(src_f, k, err, a.strip_sleep);
+                if ((uint32_t)(vf >> 32) != k) bad = true;
+            }
+            m.dfirst[jw] = (uint32_t)vf;
+        }
+        if (strip_any(bad, m.red + kRedAny, ++vote)) {
+            ok = false;
+            break;
+        }
+        // every thread's copy of the slot is in LDS: the strip may reuse it
+        if (threadIdx.x == 0)
+            __hip_atomic_store(snapc, k + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k >= 1u && !(ok = strip_publish<NT>(a, m, k - 1u))) break;
+        strip_measure<NT>(a, m, m.cb0 + wpr);
+        for (uint32_t i = threadIdx.x; i < fw; i += NT)
+            ll_put(xfirst + ((size_t)(k % fs) * nwg + w) * fw + i, m.frow[i], k + 1u);
+    }
+    if (!ok || n == 0u) return;
+    // the last sample: its first rows from below, then its publication
+    bool bad = false;
+    if (get_f)
+        m.dfirst[jw] = ll_get(xfirst + ((size_t)((n - 1u) % fs) * nwg + dn) * fw + jw, n, err, bad, a.strip_sleep);
+    if (!strip_any(bad, m.red + kRedAny, ++vote)) (void)strip_publish<NT>(a, m, n - 1u);
+}
+
 // A tail workgroup: samples q = tail, tail + n_tail, ...  For each, once every
 // strip has published it: the summed counts of levels 0..lam, the levels
 // beyond lam from the gathered level-lam rows (one per strip), the operator
@@ -808,7 +916,7 @@ __device__ void tail_body(const StripArgs &a, uint32_t *lds)
     uint32_t *ws = a.ws;
     uint32_t *err = ws + sh.o_err;
     const uint32_t rs = record_size(nl);
-    for (uint32_t q = blockIdx.x - nwg; q < a.n_sweeps; q += sh.n_tail) {
+    for (uint32_t q = blockIdx.x - nwg - sh.n_meas; q < a.n_sweeps; q += sh.n_tail) {
         const uint32_t s = q % kStripRing, gen = q / kStripRing, t = a.t0 + q, seq = q + 1u;
         const uint64_t *slot = reinterpret_cast<const uint64_t *>(ws + sh.o_ring) + (size_t)s * nwg * pw;
         for (uint32_t i = threadIdx.x; i < kStripCntWords; i += NT) cnt[i] = 0u;
@@ -875,7 +983,8 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_strip(StripArgs a)
 {
     extern __shared__ uint32_t lds[];
-    if (blockIdx.x >= a.sh.n_wg) tail_body<NT>(a, lds);
+    if (blockIdx.x >= a.sh.n_wg + a.sh.n_meas) tail_body<NT>(a, lds);
+    else if (blockIdx.x >= a.sh.n_wg) strip_body_meas<NT>(a, lds);
     else if (a.sh.halo) strip_body_halo<NT>(a, lds);
     else strip_body<NT>(a, lds);
 }
@@ -897,7 +1006,8 @@ static hipError_t launch_nt(const StripArgs &a, hipStream_t s)
     }
     StripArgs arg = a;
     void *args[] = {&arg};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_strip<NT>), dim3(a.sh.n_wg + a.sh.n_tail),
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_strip<NT>),
+                                      dim3(a.sh.n_wg + a.sh.n_meas + a.sh.n_tail),
                                       dim3(NT), args, lds_bytes, s);
 }
 

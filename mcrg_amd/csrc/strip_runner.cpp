@@ -54,7 +54,7 @@ static bool strip_enabled()
     return !(v && std::strcmp(v, "0") == 0);
 }
 
-bool StripRunner::plan(const HbmLattice &h, bool measure, StripShape &sh, int &threads)
+bool StripRunner::plan(const HbmLattice &h, bool measure, StripShape &sh, int &threads, size_t max_words)
 {
     if (h.sampler != MCRG_SAMPLER_METROPOLIS || !strip_enabled()) return false;
     const uint32_t L = (uint32_t)h.L;
@@ -80,18 +80,27 @@ bool StripRunner::plan(const HbmLattice &h, bool measure, StripShape &sh, int &t
         const int n = std::atoi(v);
         if (n == 1 || n == 2 || n == 4) want = (uint32_t)n;
     }
-    // MCRG_STRIP_HALO=0 keeps two exchanges per sweep where halo rows would fit
+    // MCRG_STRIP_HALO=0 keeps two exchanges per sweep where halo rows would
+    // fit; MCRG_STRIP_SPLIT=0 keeps a halo strip's measurements in its own
+    // workgroup
     const char *hv = std::getenv("MCRG_STRIP_HALO");
     const bool halo_ok = !(hv && std::atoi(hv) == 0);
+    const char *sv = std::getenv("MCRG_STRIP_SPLIT");
+    const bool split_ok = !(sv && std::atoi(sv) == 0);
     for (uint32_t n_tail : {4u, 2u, 1u}) {
         if (n_tail > want) continue;
-        sh = strip_shape(L, (uint32_t)h.n_levels, n_wg, n_tail, measure, halo_ok);
-        threads = strip_threads(sh);
-        // every thread gets at most one neighbour word of each kind per exchange
-        if ((sh.halo ? 4u : 2u) * sh.wpr > (uint32_t)threads || sh.first_words > (uint32_t)threads) return false;
-        const uint32_t lds = sh.lds_words * 4u;
-        if (lds > 160u * 1024u) return false;
-        if ((uint32_t)strip_max_resident(threads, lds) >= sh.n_wg + sh.n_tail) return true;
+        // measuring workgroups when they fit (resident and within max_words)
+        for (int sp = split_ok ? 1 : 0; sp >= 0; --sp) {
+            sh = strip_shape(L, (uint32_t)h.n_levels, n_wg, n_tail, measure, halo_ok, sp != 0);
+            if (sp && !sh.split) continue;
+            threads = strip_threads(sh);
+            // every thread gets at most one neighbour word of each kind per exchange
+            if ((sh.halo ? 4u : 2u) * sh.wpr > (uint32_t)threads || sh.first_words > (uint32_t)threads) return false;
+            const uint32_t lds = sh.lds_words * 4u;
+            if (lds > 160u * 1024u) return false;
+            if (max_words && sh.ws_words > max_words) continue;
+            if ((uint32_t)strip_max_resident(threads, lds) >= sh.n_wg + sh.n_meas + sh.n_tail) return true;
+        }
         if (!measure) return false;
     }
     return false;
@@ -101,7 +110,7 @@ bool StripRunner::usable(const HbmLattice &h, bool measure)
 {
     StripShape sh;
     int threads = 0;
-    return plan(h, measure, sh, threads);
+    return plan(h, measure, sh, threads, 0);
 }
 
 hipError_t StripRunner::run(const HbmLattice *d, const HbmLattice &h, uint32_t t0, uint32_t q0, uint32_t n,
@@ -109,7 +118,7 @@ hipError_t StripRunner::run(const HbmLattice *d, const HbmLattice &h, uint32_t t
 {
     StripArgs a;
     int threads = 0;
-    if (!plan(h, measure, a.sh, threads)) return hipErrorInvalidValue;
+    if (!plan(h, measure, a.sh, threads, fixed_ws ? fixed_words : 0)) return hipErrorInvalidValue;
     hipError_t e;
     if (fixed_ws) {
         if (a.sh.ws_words > fixed_words) return hipErrorInvalidValue;
@@ -143,11 +152,11 @@ hipError_t StripRunner::run(const HbmLattice *d, const HbmLattice &h, uint32_t t
     return hipSuccess;
 }
 
-size_t StripRunner::workspace_words(const HbmLattice &h, bool measure)
+size_t StripRunner::workspace_words(const HbmLattice &h, bool measure, size_t max_words)
 {
     StripShape sh;
     int threads = 0;
-    return plan(h, measure, sh, threads) ? (size_t)sh.ws_words : 0u;
+    return plan(h, measure, sh, threads, max_words) ? (size_t)sh.ws_words : 0u;
 }
 
 hipError_t StripRunner::failed(hipStream_t s, bool &out)

@@ -112,27 +112,32 @@ def test_strip_and_hbm_schedules_agree(amd, ctx, L, nl, monkeypatch):
 @pytest.mark.parametrize("L,nl", [(64, 1), (256, 4), (512, 5), (2048, 5)])
 def test_strip_halo_rows_agree(amd, ctx, orc, L, nl, monkeypatch):
     """Strips of at most 512 colour-1 words keep two halo rows a side and
-    exchange once per sweep; MCRG_STRIP_HALO=0 keeps them on two exchanges
-    per sweep.  Runs of 1, 2, 3 (unmeasured) and 5 sweeps cover the shortest
-    launches of the one-sweep-late measurement.  Both equal the oracle."""
+    exchange once per sweep; measured launches of strips of at most 320 of
+    them add a measuring workgroup per strip.  MCRG_STRIP_SPLIT=0 keeps the
+    measurements in the strip's workgroup, MCRG_STRIP_HALO=0 keeps two
+    exchanges per sweep.  Runs of 1, 2, 3 (unmeasured) and 5 sweeps cover
+    the shortest launches of the one-sweep-late measurement.  All three equal
+    the oracle."""
     seed, spb = 77 + L, 4
     _say("strip halo rows", L, nl)
     s_ref, rec_ref = orc.run_chain(L, KC, seed, 0, 1, nl, spb)
     out = []
-    for flag in ("1", "0"):
-        monkeypatch.setenv("MCRG_STRIP_HALO", flag)
+    for halo, split in (("1", "1"), ("1", "0"), ("0", "1")):
+        monkeypatch.setenv("MCRG_STRIP_HALO", halo)
+        monkeypatch.setenv("MCRG_STRIP_SPLIT", split)
+        _say("  halo", halo, "split", split)
         with amd.Chain(ctx, L, KC, seed=seed, n_levels=nl, sweeps_per_bin=spb, max_bins=3) as ch:
             ch.sweep(1, measure=True)
-            if flag == "1":
-                assert np.array_equal(ch.spins(), s_ref)
-                assert np.array_equal(ch.records()[:1], rec_ref)
+            assert np.array_equal(ch.spins(), s_ref)
+            assert np.array_equal(ch.records()[:1], rec_ref)
             ch.sweep(2, measure=True)
             ch.sweep(3)
             ch.sweep(5, measure=True)
             out.append((ch.spins(), ch.records(), ch.state()))
-    assert out[0][2] == out[1][2] == (11, 8)
-    assert np.array_equal(out[0][0], out[1][0])
-    assert np.array_equal(out[0][1], out[1][1])
+    for o in out[1:]:
+        assert o[2] == out[0][2] == (11, 8)
+        assert np.array_equal(o[0], out[0][0])
+        assert np.array_equal(o[1], out[0][1])
 
 
 @pytest.mark.parametrize("L,nl,J", [(4, 1, KC), (8, 2, KC), (16, 3, 0.42), (32, 4, KC),
