@@ -3,16 +3,19 @@
 // each strip stays bit-packed (checkerboard words) in its workgroup's LDS for
 // all the sweeps of the launch.  After each half-sweep a workgroup publishes
 // the first and last rows of the colour it updated and waits for its two
-// neighbours' rows (point-to-point flags in device memory, no grid barrier).
+// neighbours' rows (point-to-point flags in device memory, no grid barrier);
+// strips of at most 512 colour-1 words hold two halo rows a side instead and
+// exchange once per sweep (halo).
 // Measured sweeps form the level-0 operators and the blocked levels that lie
-// inside the strip (R = 2^lam rows block lam times without leaving it).  The
-// bonds between a strip's last blocked row and the next strip's first one are
-// counted by the upper strip one sweep later: the lower strip's first rows of
-// that sample travel with the next colour-0 exchange (after the last sweep of
-// a launch, in one more exchange).  Each sample's counts and each strip's
-// level-lam row (one row per strip) then go through a ring of slots to
-// `n_tail` tail workgroups of the same launch, which sum the counts, measure
-// the levels beyond lam and add the sample's record terms.
+// inside the strip (R = 2^lam rows block lam times without leaving it), in the
+// strip's workgroup or, with split, in a measuring workgroup per strip that
+// copies each sample from it.  The bonds between a strip's last blocked row
+// and the next strip's first one are counted by the upper strip later: the
+// lower strip's first rows of that sample travel with a later exchange.  Each
+// sample's counts and each strip's level-lam row (one row per strip) then go
+// through a ring of slots to `n_tail` tail workgroups of the same launch,
+// which sum the counts, measure the levels beyond lam and add the sample's
+// record terms.
 //
 // Every word that crosses workgroups travels as one 64-bit store of (data,
 // sequence number) with agent scope, and the reader polls that word until its

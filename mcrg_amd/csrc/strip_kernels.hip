@@ -6,9 +6,11 @@
 //                   launch on it (checkerboard Metropolis replacing the Wolff
 //                   step, src/ising.cpp:87-149), with the level-0 operators and
 //                   the in-strip blockings of each measured sweep
-//                   (src/lattice.cpp:102-120, src/mcrg.cpp:314-348); workgroups
-//                   n_wg .. n_wg+n_tail-1 finish each sample's operators across
-//                   strips and add its record terms (src/mcrg.cpp:72-98)
+//                   (src/lattice.cpp:102-120, src/mcrg.cpp:314-348); with
+//                   split, workgroups n_wg .. 2 n_wg-1 measure the strips'
+//                   samples instead; the last n_tail workgroups finish each
+//                   sample's operators across strips and add its record terms
+//                   (src/mcrg.cpp:72-98)
 //
 // Every random number comes from the same Philox counters as the other
 // schedules (device_common.hpp), so a chain's spins and records are
