@@ -288,11 +288,27 @@ __device__ void strip_coins(const StripArgs &a, const StripLds &m, uint32_t t, u
     }
 }
 
+// Synchronisation of the threads that run strip_levels: the workgroup's LDS
+// barrier, or for one wave (whose LDS accesses complete in order) a wave
+// barrier that keeps the compiler from moving LDS accesses across it.
+template <bool WAVE>
+__device__ __forceinline__ void group_sync()
+{
+    if (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        lds_barrier();
+    }
+}
+
 // The in-strip blocked levels 1..lam of one sample: blockings (coins from
 // m.coin), the counts of every row pair inside the strip into pend, and each
-// level's first and last row kept for the boundary bonds.
-template <int NT>
-__device__ void strip_levels(const StripArgs &a, const StripLds &m, const uint32_t *src)
+// level's first and last row kept for the boundary bonds.  Run by N threads,
+// tid = 0 .. N-1: the workgroup, or (WAVE) one wave.
+template <uint32_t N, bool WAVE>
+__device__ void strip_levels(const StripArgs &a, const StripLds &m, const uint32_t *src, uint32_t tid)
 {
     const StripShape &sh = a.sh;
     const uint32_t L = sh.L, R = sh.R, wpr = sh.wpr, sw = lg2(wpr);
@@ -301,7 +317,7 @@ __device__ void strip_levels(const StripArgs &a, const StripLds &m, const uint32
     // of rows 2by and 2by+1
     uint32_t *cur = m.lvl;
     const uint32_t *coin = m.coin;
-    for (uint32_t j = threadIdx.x; j < (R >> 1) * wpr; j += NT) {
+    for (uint32_t j = tid; j < (R >> 1) * wpr; j += N) {
         const uint32_t lb = j >> sw, qq = j & (wpr - 1u);
         const uint32_t r0 = ((1u + 2u * lb) << sw) + qq, r1 = r0 + wpr;
         const uint32_t b1 = cb0[r0], b2 = cb1[r0], b3 = cb0[r1], b4 = cb1[r1];
@@ -309,18 +325,18 @@ __device__ void strip_levels(const StripArgs &a, const StripLds &m, const uint32
         const uint32_t tie = ~(xor3(b1, b2, b3) ^ b4) & ~(b1 & b2 & b3 & b4) & (b1 | b2 | b3 | b4);
         cur[j] = maj3 | (tie & coin[j]);
     }
-    lds_barrier();
+    group_sync<WAVE>();
     for (uint32_t l = 1; l <= sh.lam; ++l) {
         const uint32_t Rl = R >> l, wl = strip_row_words(L, l), sl = lg2(wl);
         uint32_t d0 = 0, d1 = 0, d2 = 0;
-        for (uint32_t j = threadIdx.x; j < (Rl - 1u) << sl; j += NT) {
+        for (uint32_t j = tid; j < (Rl - 1u) << sl; j += N) {
             const uint32_t q = j & (wl - 1u), qn = (q + 1u) & (wl - 1u);
             const uint32_t *ar = cur + (j - q), *dr = ar + wl;
             pair_ops(ar[q], ar[qn], dr[q], dr[qn], d0, d1, d2);
         }
         // the first and last rows and the next blocking on other threads than
         // the pair counts, so that their latencies overlap
-        for (uint32_t j = (threadIdx.x + NT / 2u) % NT; j < wl; j += NT) {
+        for (uint32_t j = (tid + N / 2u) % N; j < wl; j += N) {
             m.frow[sh.row_off[l] + j] = cur[j];
             m.lrow[sh.row_off[l] + j] = cur[((Rl - 1u) << sl) + j];
         }
@@ -329,7 +345,7 @@ __device__ void strip_levels(const StripArgs &a, const StripLds &m, const uint32
             uint32_t *nxt = cur + (Rl << sl);
             coin += Rl << sl;  // level l + 1's coins follow level l's
             const uint32_t wo = wl >> 1, so = sl - 1u;
-            for (uint32_t j = NT - 1u - threadIdx.x; j < (Rl >> 1) << so; j += NT) {
+            for (uint32_t j = N - 1u - tid; j < (Rl >> 1) << so; j += N) {
                 const uint32_t jo = j >> so, qo = j & (wo - 1u);
                 const uint32_t c = coin[j];
                 uint32_t o = 0;
@@ -340,11 +356,11 @@ __device__ void strip_levels(const StripArgs &a, const StripLds &m, const uint32
                 }
                 nxt[j] = o;
             }
-            lds_barrier();
+            group_sync<WAVE>();
             cur = nxt;
         }
     }
-    lds_barrier();
+    group_sync<WAVE>();
 }
 
 // Measurement of a sample in a strip workgroup from its spins at src (both
@@ -361,13 +377,22 @@ __device__ void strip_measure(const StripArgs &a, const StripLds &m, const uint3
     // pend is zero: cleared at the start of the launch and by the previous
     // sample's publication, which ended with a barrier
     uint32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0;
-    for (uint32_t j = threadIdx.x; j < 2u * R * wpr; j += NT) {
-        const uint32_t c = j >> sc, jj = j & ((1u << sc) - 1u);
-        const uint32_t r = 1u + (jj >> sw), qq = jj & (wpr - 1u);
-        strip_ops0(src + c * m.cbs, src + (c ^ 1u) * m.cbs, wpr, r, qq, y0 + r - 1u, c, d0, d1, d2, d3);
+    // Small strips (level-1 blocking of at most 4 words per lane): the whole
+    // chain of blocked levels on wave 0, with wave barriers only, while the
+    // other waves count the level-0 operators.
+    const bool one_wave = NT >= 128 && sh.lam > 0u && (R >> 1) * wpr <= 256u;
+    if (one_wave && threadIdx.x < 64u) {
+        strip_levels<64u, true>(a, m, src, threadIdx.x);
+    } else {
+        const uint32_t t0 = one_wave ? 64u : 0u;
+        for (uint32_t j = threadIdx.x - t0; j < 2u * R * wpr; j += NT - t0) {
+            const uint32_t c = j >> sc, jj = j & ((1u << sc) - 1u);
+            const uint32_t r = 1u + (jj >> sw), qq = jj & (wpr - 1u);
+            strip_ops0(src + c * m.cbs, src + (c ^ 1u) * m.cbs, wpr, r, qq, y0 + r - 1u, c, d0, d1, d2, d3);
+        }
+        wave_add(m.pend, 4u, d0, d1, d2, d3);
     }
-    wave_add(m.pend, 4u, d0, d1, d2, d3);
-    if (sh.lam > 0u) strip_levels<NT>(a, m, src);
+    if (!one_wave && sh.lam > 0u) strip_levels<NT, false>(a, m, src, threadIdx.x);
     else lds_barrier();
 }
 
