@@ -433,6 +433,8 @@ def main():
     ap.add_argument("--max-slot-lds", type=int, default=0, help="--only batched: LDS cap per slot (0 = default)")
     ap.add_argument("--threads-large", type=int, default=0, help="--only batched: workgroup size for L = 512")
     ap.add_argument("--max-team", type=int, default=0, help="--only batched: lanes per lattice at most (0 = threads)")
+    ap.add_argument("--batch-L", type=int, default=0,
+                    help="--only batched: every replica at this L (coupling and seed as in the manifest)")
     ap.add_argument("--sub-chunks", type=int, default=0,
                     help="parts per slot and launch (0 = automatic; the autotuned run uses 0)")
     ap.add_argument("--plain-L", type=int, default=16384, help="--only plain: lattice side")
@@ -463,6 +465,8 @@ def main():
     from mcrg_amd import workloads
 
     jobs = workloads.batch_manifest(args.replicas)
+    if args.only == "batched" and args.batch_L:
+        jobs = [dict(j, L=args.batch_L, n_levels=workloads.levels_to(args.batch_L)) for j in jobs]
     ctx = mcrg_amd.Context(0)
     if args.only == "batched":
         cfg = dict(min_team=args.min_team, max_slot_lds=args.max_slot_lds, threads_large=args.threads_large,

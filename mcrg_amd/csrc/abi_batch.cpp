@@ -4,6 +4,7 @@
 // HBM schedule (int8), grouped by
 // shape so that each group advances through shared launches, on a second
 // stream that overlaps the resident launches.
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <new>
@@ -23,9 +24,17 @@ struct JobRef {
     int index;  // engine id, or member of the group
 };
 
+// Metropolis jobs of kResidentMinL <= L <= kResidentMaxL run on the resident
+// engine, the rest in int8 HBM-schedule groups.  MCRG_BATCH_RESIDENT=0 in the
+// environment sends every job to the HBM schedule (same results), to compare
+// the two schedules.
 bool resident_job(const JobSpec &s)
 {
-    return s.sampler == MCRG_SAMPLER_METROPOLIS && s.L >= kResidentMinL && s.L <= kResidentMaxL;
+    static const bool allowed = [] {
+        const char *v = std::getenv("MCRG_BATCH_RESIDENT");
+        return !(v && std::atoi(v) == 0);
+    }();
+    return allowed && s.sampler == MCRG_SAMPLER_METROPOLIS && s.L >= kResidentMinL && s.L <= kResidentMaxL;
 }
 }  // namespace
 
