@@ -16,6 +16,7 @@
 // schedules (device_common.hpp), so a chain's spins and records are
 // bit-identical to the HBM schedule's and the oracle's.
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
 
 #include "levels.hpp"
@@ -1032,10 +1033,22 @@ static hipError_t launch_nt(const StripArgs &a, hipStream_t s)
         }
     }
     StripArgs arg = a;
+    const dim3 grid(a.sh.n_wg + a.sh.n_meas + a.sh.n_tail);
+    // MCRG_STRIP_NONCOOP=1: an ordinary launch, for profilers that fail on
+    // cooperative ones.  The runner has checked that the grid fits the device
+    // at once, and on an idle device every workgroup is then dispatched; if
+    // one were not, the strips' bounded waits would report an error.
+    static const bool noncoop = [] {
+        const char *v = std::getenv("MCRG_STRIP_NONCOOP");
+        return v && std::atoi(v) != 0;
+    }();
+    if (noncoop) {
+        hipLaunchKernelGGL(k_strip<NT>, grid, dim3(NT), lds_bytes, s, arg);
+        return hipGetLastError();
+    }
     void *args[] = {&arg};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_strip<NT>),
-                                      dim3(a.sh.n_wg + a.sh.n_meas + a.sh.n_tail),
-                                      dim3(NT), args, lds_bytes, s);
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_strip<NT>), grid, dim3(NT), args, lds_bytes,
+                                      s);
 }
 
 hipError_t launch_strip(const StripArgs &a, int threads, hipStream_t s)
