@@ -703,6 +703,7 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
     const uint32_t yc = (y0 + jr2 + L - 1u) & (L - 1u);
     const uint32_t row0 = (2u + jr) << sw, row1 = (1u + jr) << sw, row2 = (1u + jr2) << sw;
     u32x4 p0{0u, 0u, 0u, 0u}, p0r{0u, 0u, 0u, 0u}, p1{0u, 0u, 0u, 0u}, p1r{0u, 0u, 0u, 0u};
+    u32x4 p2{0u, 0u, 0u, 0u}, p2r{0u, 0u, 0u, 0u};  // second words: 256-thread launches only
     if (has0 && n > 0u) metropolis_rng(ya * wpr + q, a.t0, 0u, k0, k1, p0, p0r);
     // neighbour words: local rows 0, 1 from the upper strip's last two rows,
     // R+2, R+3 from the lower strip's first two (one word per thread)
@@ -754,6 +755,7 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
                 ll_put(xfirst + ((size_t)((it - 1u) % fs) * nwg + w) * fw + i, m.frow[i], it);
         }
         if (has1) metropolis_rng(yb * wpr + q, t, 1u, k0, k1, p1, p1r);
+        if (NT == 256 && has2) metropolis_rng(yc * wpr + q2, t, 1u, k0, k1, p2, p2r);
         if (has0 && it + 1u < n) metropolis_rng(ya * wpr + q, t + 1u, 0u, k0, k1, p0, p0r);
         if (own && sh.lam > 0u) strip_coins<NT>(a, m, t, k0, k1);
         if (get_h) {
@@ -768,13 +770,7 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
             break;
         }
         if (has1) cb1[row1 + q] ^= strip_flip(cb1, cb0, row1, q, wpr, yb, 1u, acc, k0, k1, t, p1, p1r);
-        if (NT == 256 && has2) {
-            // a second word (measuring-workgroup launches only): its Philox
-            // calls drawn here
-            u32x4 p2, p2r;
-            metropolis_rng(yc * wpr + q2, t, 1u, k0, k1, p2, p2r);
-            cb1[row2 + q2] ^= strip_flip(cb1, cb0, row2, q2, wpr, yc, 1u, acc, k0, k1, t, p2, p2r);
-        }
+        if (NT == 256 && has2) cb1[row2 + q2] ^= strip_flip(cb1, cb0, row2, q2, wpr, yc, 1u, acc, k0, k1, t, p2, p2r);
         lds_barrier();
         if (own) {
             // sample t, measured during the next sweep's exchange
