@@ -769,30 +769,42 @@ __device__ void strip_body_halo(const StripArgs &a, uint32_t *lds)
             ok = false;
             break;
         }
-        if (has1) cb1[row1 + q] ^= strip_flip(cb1, cb0, row1, q, wpr, yb, 1u, acc, k0, k1, t, p1, p1r);
-        if (NT == 256 && has2) cb1[row2 + q2] ^= strip_flip(cb1, cb0, row2, q2, wpr, yc, 1u, acc, k0, k1, t, p2, p2r);
-        lds_barrier();
-        if (own) {
-            // sample t, measured during the next sweep's exchange
-            for (uint32_t i = threadIdx.x; i < 2u * cbs; i += NT) m.snap[i] = cb0[i];
-            lds_barrier();
-        } else if (sh.split) {
-            // sample t to the measuring workgroup: both colours of local rows
-            // 1 .. R+2; a failed wait is voted on at the next barrier
+        // With split, sample t goes to the measuring workgroup as each word
+        // is final: both colours of local rows 1 .. R+2, word i of a colour
+        // at slot index i (colour 1 after the first half), written by the
+        // thread that updates colour-1 word i.  Colour 0 is final once the
+        // neighbours' rows are in; colour 1 goes from the register that
+        // computed it.  A failed wait for the slot is voted on at the next
+        // barrier.
+        uint64_t *dst = nullptr;
+        if (sh.split) {
             copied = copied_now;
             if (copied < need) {
                 copied = flag_wait(snapc, need, err) ? need : 0u;
                 bad = copied < need;
             }
             if (!bad) {
-                uint64_t *dst = snap + ((size_t)(it % kStripSnapSlots) * nwg + w) * sh.snap_words;
-                const uint32_t half = sh.snap_words >> 1;
-                for (uint32_t i = threadIdx.x; i < sh.snap_words; i += NT) {
-                    const uint32_t c = i >= half ? 1u : 0u;
-                    ll_put(dst + i, cb0[c * cbs + wpr + (i - c * half)], seq);
-                }
+                dst = snap + ((size_t)(it % kStripSnapSlots) * nwg + w) * sh.snap_words;
+                if (has1) ll_put(dst + jw, cb0[row1 + q], seq);
+                if (NT == 256 && has2) ll_put(dst + j2, cb0[row2 + q2], seq);
             }
-            lds_barrier();  // the copy is read before the next colour-0 update
+        }
+        const uint32_t half = sh.snap_words >> 1;
+        if (has1) {
+            const uint32_t v = cb1[row1 + q] ^ strip_flip(cb1, cb0, row1, q, wpr, yb, 1u, acc, k0, k1, t, p1, p1r);
+            cb1[row1 + q] = v;
+            if (dst) ll_put(dst + half + jw, v, seq);
+        }
+        if (NT == 256 && has2) {
+            const uint32_t v = cb1[row2 + q2] ^ strip_flip(cb1, cb0, row2, q2, wpr, yc, 1u, acc, k0, k1, t, p2, p2r);
+            cb1[row2 + q2] = v;
+            if (dst) ll_put(dst + half + j2, v, seq);
+        }
+        lds_barrier();
+        if (own) {
+            // sample t, measured during the next sweep's exchange
+            for (uint32_t i = threadIdx.x; i < 2u * cbs; i += NT) m.snap[i] = cb0[i];
+            lds_barrier();
         }
     }
     if (ok && sh.split && strip_any(bad, m.red + kRedAny, ++vote)) ok = false;
