@@ -656,9 +656,11 @@ __device__ __forceinline__ uint32_t strip_flip(const uint32_t *own, const uint32
 //     whose decisions for them come from the same counters, so afterwards
 //     rows 1 .. R+2 hold sample t whole and the next colour-0 update needs no
 //     exchange;
-//   * when measuring, the copy of sample t: in LDS, or with split to the
-//     strip's measuring workgroup (slot t mod kStripSnapSlots, once that
-//     workgroup has copied the slot's previous sample).
+//   * when measuring, the copy of sample t: in LDS after colour 1, or with
+//     split to the strip's measuring workgroup (slot t mod kStripSnapSlots,
+//     once that workgroup has copied the slot's previous sample), each word
+//     as soon as it is final: colour 0 after the exchange, colour 1 from the
+//     update's registers.
 // A sample's first blocked rows are read one sweep after they are sent, from
 // first_slots slots per strip: a strip may send the next two samples' before
 // its upper neighbour has read them.
