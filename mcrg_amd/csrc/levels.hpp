@@ -161,16 +161,15 @@ __device__ __forceinline__ Nbr4 cb_neighbours(const uint32_t *oth, uint32_t L, u
         const uint32_t y = w >> sh, q = w & (wpr - 1u);
         const uint32_t p = (y + c) & 1u;
         const uint32_t row = y << sh;
+        // the word beside o that holds the row's wrap-around neighbour: word
+        // q - 1 on p = 0 rows, q + 1 on p = 1 rows (one load either way, so
+        // that every load of the word issues before one wait)
         const uint32_t o = oth[row + q];
+        const uint32_t side = oth[row + ((q + 2u * p - 1u) & (wpr - 1u))];
         n.nu = oth[(((y - 1u) & (L - 1u)) << sh) + q];
         n.nd = oth[(((y + 1u) & (L - 1u)) << sh) + q];
-        if (p == 0u) {
-            n.nr = o;
-            n.nl = (o << 1) | (oth[row + ((q - 1u) & (wpr - 1u))] >> 31);
-        } else {
-            n.nl = o;
-            n.nr = (o >> 1) | (oth[row + ((q + 1u) & (wpr - 1u))] << 31);
-        }
+        n.nr = p == 0u ? o : (o >> 1) | (side << 31);
+        n.nl = p == 0u ? (o << 1) | (side >> 31) : o;
     } else {
         const SegMasks m = seg_masks(L);
         const uint32_t o = oth[w];
@@ -201,13 +200,9 @@ __device__ __forceinline__ void cb_ops_item(const uint32_t *own, const uint32_t 
         const uint32_t p = (y + c) & 1u;
         const uint32_t row = ((y + 1u) & (L - 1u)) << sh;
         const uint32_t nx = own[row + q];
-        if (p == 0u) {
-            dr = nx;
-            dl = (nx << 1) | (own[row + ((q - 1u) & (wpr - 1u))] >> 31);
-        } else {
-            dl = nx;
-            dr = (nx >> 1) | (own[row + ((q + 1u) & (wpr - 1u))] << 31);
-        }
+        const uint32_t side = own[row + ((q + 2u * p - 1u) & (wpr - 1u))];
+        dr = p == 0u ? nx : (nx >> 1) | (side << 31);
+        dl = p == 0u ? (nx << 1) | (side >> 31) : nx;
     } else {
         const SegMasks m = seg_masks(L);
         const uint32_t sn = own[(w + 1u) & (W - 1u)];
