@@ -201,6 +201,26 @@ def test_sampler_samples_exact_4x4_ensemble(orc, J, sampler):
     assert abs(m2.mean() - ex["M2"]) < 5 * err
 
 
+@pytest.mark.parametrize("J", [0.1, KC])
+def test_uphill_flip_rate_is_the_threshold(orc, J):
+    """From the all-up lattice every colour-0 site has e = 4, so one colour-0
+    half-sweep flips each with probability exactly thr[1] / 2^32, through the
+    4 top bits and, for the 1/16 of sites they leave open, the refinement
+    bits.  A comparison that mishandles the open sites (accepting or rejecting
+    them all, or reading the wrong threshold bits) moves the rate by a
+    sizeable fraction of 1/16, many standard errors at this sample size."""
+    L, n_t = 64, 600
+    p = float(orc.thresholds(J)[1]) / 2.0**32
+    flips = 0
+    for t in range(n_t):
+        s = np.ones(L * L, dtype=np.int8)
+        orc.half_sweep(L, s, 7, J, t, 0)
+        flips += int((s < 0).sum())
+    n = n_t * L * L // 2
+    err = math.sqrt(p * (1.0 - p) / n)
+    assert abs(flips / n - p) < 5 * err, (flips / n, p, err)
+
+
 def test_rg_matrix_converges_to_exact_4x4(orc):
     """The whole measurement chain (operators, blocking with coins, record layout,
     connected correlators, T = A^-1 B, leading eigenvalue) reproduces the exact
