@@ -79,14 +79,14 @@ MEASURED_TRAFFIC = {
 # issue one wave64 VALU instruction per 4 cycles).  Measured on the kernel with
 # the wide slot body; parts add only their reloads.
 MEASURED_VALU = {
-    (4096, 100, 256, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 4.700e8, "gui_active_cycles_per_xcd": 2.655e7 / 8},
-    (4096, 100, 256, 16, 0, 512, 1): {"valu_wave_instr_per_launch": 4.700e8, "gui_active_cycles_per_xcd": 2.655e7 / 8},
-    (4096, 100, 256, 32, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6999e8, "gui_active_cycles_per_xcd": 2.681e7 / 8},
-    (4096, 100, 256, 8, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6999e8, "gui_active_cycles_per_xcd": 2.6785e7 / 8},
-    (4096, 100, 128, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 5.0055e8, "gui_active_cycles_per_xcd": 3.5332e7 / 8},
-    (4096, 100, 512, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 4.7003e8, "gui_active_cycles_per_xcd": 3.8077e7 / 8},
-    (4096, 100, 256, 16, 65536, 0, 1): {"valu_wave_instr_per_launch": 2.0 * 2.3242e8,
-                                        "gui_active_cycles_per_xcd": 2.5244e7 / 8},
+    (4096, 100, 256, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6724e8, "gui_active_cycles_per_xcd": 2.4876e7 / 8},
+    (4096, 100, 256, 16, 0, 512, 1): {"valu_wave_instr_per_launch": 4.6724e8, "gui_active_cycles_per_xcd": 2.4876e7 / 8},
+    (4096, 100, 256, 32, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6724e8, "gui_active_cycles_per_xcd": 2.5067e7 / 8},
+    (4096, 100, 256, 8, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6724e8, "gui_active_cycles_per_xcd": 2.4750e7 / 8},
+    (4096, 100, 128, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 4.9458e8, "gui_active_cycles_per_xcd": 3.4708e7 / 8},
+    (4096, 100, 512, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6728e8, "gui_active_cycles_per_xcd": 3.7474e7 / 8},
+    (4096, 100, 256, 16, 65536, 0, 1): {"valu_wave_instr_per_launch": 2.0 * 2.3104e8,
+                                        "gui_active_cycles_per_xcd": 2.4222e7 / 8},
 }
 
 
