@@ -496,9 +496,6 @@ __device__ void run_slot(const Slot sl, const int32_t *__restrict__ slot_lats,
         // The sample's sums and products into lattice g's partial record,
         // entries k = k0, k0 + ks, ...: from the operator values in `sv`, or
         // each entry forming its two values from the counts.
-        // The sample's sums and products into lattice g's partial record,
-        // entries k = k0, k0 + ks, ...: from the operator values in `sv`, or
-        // each entry forming its two values from the counts.
         auto record_sample = [&](uint32_t g, uint32_t k0, uint32_t ks, bool use_sv) {
             const uint32_t *m = meta + g * kMetaWords;
             const uint32_t q = m[kMetaNMeas] + (it - m[kMetaTherm]);
