@@ -238,16 +238,11 @@ __global__ __launch_bounds__(256) void k_int8_half_sweep(const HbmLattice *__res
             const uint32_t o = gather_parity(C, p ^ 1u);
             const uint32_t nu = gather_parity(U, p);
             const uint32_t nd = gather_parity(D, p);
-            uint32_t nl, nr;
-            if (p == 0u) {
-                const uint32_t e = sbit(rowbase[(64u * q - 1u) & (L - 1u)]);
-                nr = o;
-                nl = (o << 1) | e;
-            } else {
-                const uint32_t e = sbit(rowbase[(64u * q + 64u) & (L - 1u)]);
-                nl = o;
-                nr = (o >> 1) | (e << 31);
-            }
+            // the wrap-around site's byte: the last of the chunk before on
+            // p = 0 rows, the first of the chunk after on p = 1 rows
+            const uint32_t e = sbit(rowbase[(64u * q + 65u * p - 1u) & (L - 1u)]);
+            const uint32_t nr = p == 0u ? o : (o >> 1) | (e << 31);
+            const uint32_t nl = p == 0u ? (o << 1) | e : o;
             const uint32_t flip = metropolis_word(s, nu, nd, nl, nr, 0xFFFFFFFFu, acc, lat.k0, lat.k1,
                                                   y * wpr + q, t, colour);
             if (flip) {

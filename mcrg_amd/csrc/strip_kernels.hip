@@ -165,18 +165,14 @@ __device__ __forceinline__ void strip_ops0(const uint32_t *own, const uint32_t *
     const uint32_t s = own[row + q];
     const uint32_t nu = oth[row - wpr + q], nd = oth[row + wpr + q], o = oth[row + q];
     const uint32_t nx = own[row + wpr + q];
-    uint32_t nl, nr, dl, dr;
-    if (((y + c) & 1u) == 0u) {
-        nr = o;
-        nl = (o << 1) | (oth[row + ((q - 1u) & m)] >> 31);
-        dr = nx;
-        dl = (nx << 1) | (own[row + wpr + ((q - 1u) & m)] >> 31);
-    } else {
-        nl = o;
-        nr = (o >> 1) | (oth[row + ((q + 1u) & m)] << 31);
-        dl = nx;
-        dr = (nx >> 1) | (own[row + wpr + ((q + 1u) & m)] << 31);
-    }
+    // the words beside o and nx that hold the wrap-around sites: q - 1 on
+    // p = 0 rows, q + 1 on p = 1 rows (one load each, before any wait)
+    const uint32_t p = (y + c) & 1u, qs = (q + 2u * p - 1u) & m;
+    const uint32_t os = oth[row + qs], nxs = own[row + wpr + qs];
+    const uint32_t nr = p == 0u ? o : (o >> 1) | (os << 31);
+    const uint32_t nl = p == 0u ? (o << 1) | (os >> 31) : o;
+    const uint32_t dr = p == 0u ? nx : (nx >> 1) | (nxs << 31);
+    const uint32_t dl = p == 0u ? (nx << 1) | (nxs >> 31) : nx;
     // every nearest-neighbour bond has one colour-0 end
     if (c == 0u) dnn += (uint32_t)(popc(s ^ nu) + popc(s ^ nd) + popc(s ^ nl) + popc(s ^ nr));
     dnnn += (uint32_t)(popc(s ^ dr) + popc(s ^ dl));
@@ -521,15 +517,12 @@ __device__ void strip_body(const StripArgs &a, uint32_t *lds)
             const uint32_t *oth = cb0 + (c ^ 1u) * cbs;
             for (uint32_t j = threadIdx.x; j < R * wpr; j += NT) {
                 const uint32_t r = 1u + (j >> sw), q = j & mk, y = y0 + r - 1u, row = r << sw;
-                const uint32_t o = oth[row + q];
-                uint32_t nl, nr;
-                if (((y + c) & 1u) == 0u) {
-                    nr = o;
-                    nl = (o << 1) | (oth[row + ((q - 1u) & mk)] >> 31);
-                } else {
-                    nl = o;
-                    nr = (o >> 1) | (oth[row + ((q + 1u) & mk)] << 31);
-                }
+                // o and the word beside it that holds the wrap-around site
+                // (q - 1 on p = 0 rows, q + 1 on p = 1 rows): one load each
+                const uint32_t p = (y + c) & 1u;
+                const uint32_t o = oth[row + q], os = oth[row + ((q + 2u * p - 1u) & mk)];
+                const uint32_t nr = p == 0u ? o : (o >> 1) | (os << 31);
+                const uint32_t nl = p == 0u ? (o << 1) | (os >> 31) : o;
                 if (one)
                     own[row + q] ^= metropolis_word_drawn(own[row + q], oth[row - wpr + q], oth[row + wpr + q], nl, nr,
                                                           acc, k0, k1, y * wpr + q, t, c, pr, pref);
@@ -628,15 +621,12 @@ __device__ __forceinline__ uint32_t strip_flip(const uint32_t *own, const uint32
                                                uint32_t wpr, uint32_t y, uint32_t c, const Accept &acc, uint32_t k0,
                                                uint32_t k1, uint32_t t, const u32x4 &r, const u32x4 &ref)
 {
-    const uint32_t mk = wpr - 1u, o = oth[row + q];
-    uint32_t nl, nr;
-    if (((y + c) & 1u) == 0u) {
-        nr = o;
-        nl = (o << 1) | (oth[row + ((q - 1u) & mk)] >> 31);
-    } else {
-        nl = o;
-        nr = (o >> 1) | (oth[row + ((q + 1u) & mk)] << 31);
-    }
+    // o and the word beside it that holds the wrap-around site (q - 1 on
+    // p = 0 rows, q + 1 on p = 1 rows): one load each, before any wait
+    const uint32_t mk = wpr - 1u, p = (y + c) & 1u;
+    const uint32_t o = oth[row + q], os = oth[row + ((q + 2u * p - 1u) & mk)];
+    const uint32_t nr = p == 0u ? o : (o >> 1) | (os << 31);
+    const uint32_t nl = p == 0u ? (o << 1) | (os >> 31) : o;
     return metropolis_word_drawn(own[row + q], oth[row - wpr + q], oth[row + wpr + q], nl, nr, acc, k0, k1,
                                  y * wpr + q, t, c, r, ref);
 }
