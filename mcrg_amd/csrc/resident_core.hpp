@@ -79,13 +79,21 @@ __device__ __forceinline__ uint32_t group_sum(uint32_t v, uint32_t width)
     return v;
 }
 
+// The unmeasured and measured sweeps left (kMetaTherm, kMetaMeas: adjacent
+// words, 8-byte aligned in each lattice's 64-byte meta block) in one LDS read.
+__device__ __forceinline__ uint2 sweeps_left(const uint32_t *m)
+{
+    return *reinterpret_cast<const uint2 *>(m + kMetaTherm);
+}
 __device__ __forceinline__ bool in_run(const uint32_t *m, uint32_t it)
 {
-    return it < m[kMetaTherm] + m[kMetaMeas];
+    const uint2 v = sweeps_left(m);
+    return it < v.x + v.y;
 }
 __device__ __forceinline__ bool in_meas(const uint32_t *m, uint32_t it)
 {
-    return it >= m[kMetaTherm] && it < m[kMetaTherm] + m[kMetaMeas];
+    const uint2 v = sweeps_left(m);
+    return it >= v.x && it < v.x + v.y;
 }
 
 // Operator counts of lattice g at level l: sum over the team's wave partials;
@@ -318,9 +326,12 @@ __device__ __forceinline__ void pyramid_t(const uint32_t *lg, const uint32_t *cg
         const uint32_t Ll = LP >> j;
         const bool wide = Ll == 64u;
         const uint32_t Lb = Ll >> 1;
+        // the coin word of the blocks lane y forms (lane 2by forms blocked row
+        // by), loaded by every lane (rows y >= Ll wrap into the level) under a
+        // wave-uniform test, so that its wait comes at the blocking below
         uint32_t coin = 0;
-        if (l < nl && y < Ll && (y & 1u) == 0u) {
-            const uint32_t bm = (y >> 1) * Lb;
+        if (l < nl) {
+            const uint32_t bm = ((y & (Ll - 1u)) >> 1) * Lb;
             coin = cg[coff + (bm >> 5)] >> (bm & 31u);
         }
         const uint32_t dlo = row_next(lo, y, Ll, WIDTH);
