@@ -76,13 +76,15 @@ MEASURED_TRAFFIC = {
 # The same kernel is bound by VALU issue, not HBM: rocprofv3 --pmc SQ_INSTS_VALU
 # and GRBM_GUI_ACTIVE over the same one-part launches give the wave-level VALU
 # instructions per launch and the busy cycles per XCD (8 XCDs, 1024 SIMDs that
-# issue one wave64 VALU instruction per 4 cycles).  Measured on the kernel with
-# the wide slot body; parts add only their reloads.
+# issue one wave64 VALU instruction per 4 cycles); parts add only their
+# reloads.  The 128- and 512-thread shapes and the 64 KiB cap were counted on
+# a kernel that read each lattice's two sweep counters with two loads (about
+# 0.5 % more VALU instructions than this one).
 MEASURED_VALU = {
-    (4096, 100, 256, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6724e8, "gui_active_cycles_per_xcd": 2.4876e7 / 8},
-    (4096, 100, 256, 16, 0, 512, 1): {"valu_wave_instr_per_launch": 4.6724e8, "gui_active_cycles_per_xcd": 2.4876e7 / 8},
-    (4096, 100, 256, 32, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6724e8, "gui_active_cycles_per_xcd": 2.5067e7 / 8},
-    (4096, 100, 256, 8, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6724e8, "gui_active_cycles_per_xcd": 2.4750e7 / 8},
+    (4096, 100, 256, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6509e8, "gui_active_cycles_per_xcd": 2.4379e7 / 8},
+    (4096, 100, 256, 16, 0, 512, 1): {"valu_wave_instr_per_launch": 4.6509e8, "gui_active_cycles_per_xcd": 2.4379e7 / 8},
+    (4096, 100, 256, 32, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6509e8, "gui_active_cycles_per_xcd": 2.4261e7 / 8},
+    (4096, 100, 256, 8, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6509e8, "gui_active_cycles_per_xcd": 2.4248e7 / 8},
     (4096, 100, 128, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 4.9458e8, "gui_active_cycles_per_xcd": 3.4708e7 / 8},
     (4096, 100, 512, 16, 0, 0, 1): {"valu_wave_instr_per_launch": 4.6728e8, "gui_active_cycles_per_xcd": 3.7474e7 / 8},
     (4096, 100, 256, 16, 65536, 0, 1): {"valu_wave_instr_per_launch": 2.0 * 2.3104e8,
