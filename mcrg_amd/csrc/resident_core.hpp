@@ -81,6 +81,8 @@ __device__ __forceinline__ uint32_t group_sum(uint32_t v, uint32_t width)
 
 // The unmeasured and measured sweeps left (kMetaTherm, kMetaMeas: adjacent
 // words, 8-byte aligned in each lattice's 64-byte meta block) in one LDS read.
+static_assert(kMetaMeas == kMetaTherm + 1u && kMetaTherm % 2u == 0u && kMetaWords % 2u == 0u,
+              "sweeps_left reads kMetaTherm and kMetaMeas as one aligned 64-bit word");
 __device__ __forceinline__ uint2 sweeps_left(const uint32_t *m)
 {
     return *reinterpret_cast<const uint2 *>(m + kMetaTherm);
