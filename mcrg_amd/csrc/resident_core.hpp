@@ -330,7 +330,7 @@ __device__ __forceinline__ void pyramid_t(const uint32_t *lg, const uint32_t *cg
         const uint32_t Lb = Ll >> 1;
         // the coin word of the blocks lane y forms (lane 2by forms blocked row
         // by), loaded by every lane (rows y >= Ll wrap into the level) under a
-        // wave-uniform test, so that its wait comes at the blocking below
+        // wave-uniform test instead of a lane-divergent one
         uint32_t coin = 0;
         if (l < nl) {
             const uint32_t bm = ((y & (Ll - 1u)) >> 1) * Lb;
